@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the 13-field Biot time step on one MI355X.
+
+Workload (default): 256^3 grid, 4th-order staggered space, PML 20, three-layer porous
+medium, 20 Hz Ricker at the centre, 65,536 surface receivers recording vx, vz, p; FP32
+storage (the drop-in path), 2000 steps timed after a warm-up.  The same grid is then run
+with FP16 storage.  A "step" is one time step over the whole grid (source, velocity half
+step, stress half step, receiver sampling), inputs resident in HBM.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "mixed-precision-optimization-method_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "Mcell-updates/s for the 13-field Biot step, FP16 vs FP32, plus adjoint grad ms/shot"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def slab_counts(n, L, R, h):
+    """Updated cells (h..n-1-h) that lie in the low/high absorbing slab of one axis."""
+    lo = max(0, min(L, n - h) - h)
+    hi = max(0, (n - h) - max(n - R, h))
+    return lo + hi
+
+
+def algorithmic_bytes(nx, ny, nz, order, npml_cells, w):
+    """Compulsory HBM bytes per launch of the velocity and stress kernels and per step.
+
+    Velocity kernel per updated cell: reads 7 stress/pressure + 6 velocity fields, writes 6,
+    reads 14 coefficient arrays; in an absorbing slab of one axis it reads and writes 4
+    carries.  Stress kernel: reads 6 + 7, writes 7, reads 8 coefficients, 4 carries per
+    slab.  The fused-step figure is 26 field + 22 coefficient accesses per cell plus
+    16 carry accesses per slab membership (one read and one write of 8 carries)."""
+    h = order // 2
+    ni, nj, nk = nx - 2 * h, ny - 2 * h, nz - 2 * h
+    upd = ni * nj * nk
+    Lx, Rx, Ly, Ry, Lz, Rz = npml_cells
+    memb = (slab_counts(nx, Lx, Rx, h) * nj * nk + slab_counts(ny, Ly, Ry, h) * ni * nk +
+            slab_counts(nz, Lz, Rz, h) * ni * nj)
+    v = upd * (19 + 14) * w + memb * 8 * w
+    t = upd * (20 + 8) * w + memb * 8 * w
+    step = upd * (26 + 22) * w + memb * 16 * w
+    return v, t, step, upd, memb
+
+
+def slab_widths(s):
+    """(L, R) per axis from the PML profile the handle holds."""
+    out = []
+    for ax, n in enumerate((s.nx, s.ny, s.nz)):
+        d, d2, _, _ = s.pml_profile(ax)
+        nzr = [(d[i] != 0 or d2[i] != 0) for i in range(n)]
+        L = max([i + 1 for i in range(n // 2) if nzr[i]], default=0)
+        R = max([n - i for i in range(n // 2, n) if nzr[i]], default=0)
+        out += [L, R]
+    return tuple(out)
+
+
+def run_precision(cfg, precision, steps, warmup):
+    from biot_amd import configs
+
+    nt = warmup + steps
+    t0 = time.time()
+    s = configs.make_solver(cfg, precision=precision, nt=nt)
+    log(f"[bench] {precision}: set-up {time.time() - t0:.1f} s, device {s.device_bytes() / 2**30:.2f} GiB")
+    s.run(warmup)  # biot_run returns after the device has finished
+    t0 = time.perf_counter()
+    s.run(steps)
+    wall = time.perf_counter() - t0
+    ev_ms = s.last_run_ms()
+    tr = s.traces()
+    import numpy as np
+
+    finite = bool(np.isfinite(tr).all())
+    # per-kernel durations over a separate profiled stretch (events around each launch)
+    pv, pt = s.profile_steps(min(100, steps))
+    widths = slab_widths(s)
+    s.close()
+    return dict(wall_s=wall, event_ms=ev_ms, kernel_ms_v=pv, kernel_ms_t=pt, widths=widths,
+                finite=finite)
+
+
+def cpu_baseline(cfg, nsteps):
+    """The oracle (the reference update arithmetic restated in C, OpenMP over z-planes)
+    timed on the host cores over a bounded number of steps of the same workload."""
+    sys.path.insert(0, ROOT)
+    import numpy as np
+
+    import oracle
+    from biot_amd import configs
+
+    oracle.build()
+    o = oracle.OracleRun(cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["dt"], order=cfg["order"],
+                         mode="f32", npml=cfg["npml"], medium=cfg["medium"])
+    w = oracle.ricker_series(nsteps, cfg["dt"], cfg["f0"])
+    rec = configs.cells(cfg, cfg["rec"])
+    t0 = time.perf_counter()
+    o.run(0, nsteps, nsteps, w, configs.cells(cfg, cfg["src"])[0], cfg["src_fields"], rec,
+          cfg["rec_fields"])
+    dt = time.perf_counter() - t0
+    n = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    o.close()
+    return dict(value=n * nsteps / dt / 1e6, unit="Mcell-updates/s", cores=oracle.num_threads(),
+                kind="port",
+                sample=f"{nsteps} steps of the same 256^3 order-4 FP32 workload, oracle in C "
+                       f"(-O3, OpenMP), {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the solver holds the whole grid on a single MI355X "
+              f"and has no multi-GPU path; got --gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+
+    from biot_amd import configs
+
+    cfg = configs.layered256(order=4, nt=args.warmup + args.steps, n=args.size)
+    n = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    r32 = run_precision(cfg, "fp32", args.steps, args.warmup)
+    value = n * args.steps / r32["wall_s"] / 1e6
+    log(f"[bench] fp32: {value:.0f} Mcell/s, V {r32['kernel_ms_v']:.3f} ms T {r32['kernel_ms_t']:.3f} ms")
+    bv, bt, bstep, upd, memb = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"],
+                                                 r32["widths"], 4)
+    ach_v = bv / (r32["kernel_ms_v"] * 1e-3) / 1e9
+    ach_t = bt / (r32["kernel_ms_t"] * 1e-3) / 1e9
+    ach_step = bstep / ((r32["kernel_ms_v"] + r32["kernel_ms_t"]) * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"layered{args.size}: {cfg['nx']}^3 cells, order 4, PML 20, "
+                               f"3-layer Biot medium, 20 Hz Ricker, {len(cfg['rec'])} receivers x "
+                               f"(vx, vz, p), FP32 storage",
+                   "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "order": 4, "npml": 20,
+                   "dx_m": cfg["dx"], "dt_s": cfg["dt"]},
+        "roofline": {"bound": "hbm", "kernel": "k_velocity<float,4>",
+                     "achieved": round(ach_v, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach_v / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": bv,
+                     "mean_launch_ms": round(r32["kernel_ms_v"], 5)},
+        "stress_kernel": {"kernel": "k_stress<float,4>", "achieved_GBs": round(ach_t, 1),
+                          "frac": round(ach_t / HBM_PEAK_GBS, 4), "algorithmic_bytes": bt,
+                          "mean_launch_ms": round(r32["kernel_ms_t"], 5)},
+        "step_roofline": {"achieved_GBs": round(ach_step, 1), "frac": round(ach_step / HBM_PEAK_GBS, 4),
+                          "algorithmic_bytes_per_step": bstep,
+                          "bytes_per_cell_update": round(bstep / upd, 2)},
+        "finite": r32["finite"],
+    }
+    if not args.no_fp16:
+        r16 = run_precision(cfg, "fp16", args.steps, args.warmup)
+        v16 = n * args.steps / r16["wall_s"] / 1e6
+        bv16, bt16, bstep16, _, _ = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"],
+                                                      r16["widths"], 2)
+        a16 = bstep16 / ((r16["kernel_ms_v"] + r16["kernel_ms_t"]) * 1e-3) / 1e9
+        log(f"[bench] fp16: {v16:.0f} Mcell/s ({v16 / value:.2f}x fp32)")
+        out["fp16"] = {"value": round(v16, 1), "unit": "Mcell-updates/s",
+                       "speedup_vs_fp32": round(v16 / value, 3),
+                       "ms_per_step": round(r16["wall_s"] * 1e3 / args.steps, 4),
+                       "step_achieved_GBs": round(a16, 1),
+                       "step_frac": round(a16 / HBM_PEAK_GBS, 4),
+                       "kernel_ms_v": round(r16["kernel_ms_v"], 5),
+                       "kernel_ms_t": round(r16["kernel_ms_t"], 5), "finite": r16["finite"]}
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
+        log(f"[bench] cpu: {out['cpu_baseline']['value']:.1f} Mcell/s on {out['cpu_baseline']['cores']} threads")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+/*
+ * biot.h - C-ABI of the MI355X (gfx950) solver for the 13-field staggered-grid Biot
+ * (poroelastic) velocity-stress system with C-PML, source injection and receivers.
+ *
+ * This ABI replaces the reference's single entry point
+ *     void callCUDAKernel(blockInfo *parameter, int nxt_pro, int nyt_pro, int nzt_pro,
+ *                         float DT, float F0, int sn, int *coord, int myid, int up, ...,
+ *                         int sxnew, int synew, int sznew, int NT)      (kernel.cu:1728)
+ * and the 99 caller-filled device arrays of `blockInfo` (common.h:6-133, filled in
+ * main.cpp:1376-1743) with an opaque handle that owns its device memory.  The MPI
+ * arguments (coord, myid, up..right, MCW) have no counterpart: one GPU holds the grid.
+ *
+ * Conventions
+ *   - Every call returns BIOT_OK (0) or a BIOT_ERR_* code; nothing exits or aborts
+ *     (the reference's CUDA_CHECK -> exit(EXIT_FAILURE), kernel.cu:42-52, is replaced).
+ *     biot_last_error() gives the message of the last failure on a handle.
+ *   - Arrays are host pointers, x fastest: cell (i, j, k) is i + j*nx + k*nx*ny
+ *     (main.cpp:1489).  Values are in the reference representation: velocities scaled
+ *     by Cv, stresses and pressure by Cp (main.cpp:248-249).
+ *   - One handle is used by one host thread at a time; distinct handles are independent.
+ *
+ * Field numbering: 0 vx 1 vy 2 vz 3 qx 4 qy 5 qz 6 txx 7 tyy 8 tzz 9 txy 10 txz 11 tyz 12 p
+ * (reference names vux vuy vuz vwx vwy vwz txx tyy tzz txy txz tyz ss).
+ */
+#ifndef BIOT_H
+#define BIOT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BIOT_ABI_VERSION 1
+
+enum biot_status {
+  BIOT_OK = 0,
+  BIOT_ERR_ARG = 1,        /* invalid argument (dims, order, indices, sizes) */
+  BIOT_ERR_HIP = 2,        /* HIP runtime failure */
+  BIOT_ERR_STATE = 3,      /* call out of order (e.g. run before set_medium) */
+  BIOT_ERR_NONFINITE = 4,  /* NaN/Inf detected on the receivers */
+  BIOT_ERR_RANGE = 5,      /* value outside the storage format's range (FP16 scaling) */
+  BIOT_ERR_NOMEM = 6       /* device or host allocation failed */
+};
+
+enum biot_precision {
+  BIOT_FP32 = 0,  /* float storage, float arithmetic: the drop-in path */
+  BIOT_FP16 = 1,  /* IEEE binary16 storage with per-field scaling, float arithmetic */
+  BIOT_FP64 = 2   /* double storage and arithmetic (accuracy reference on the GPU) */
+};
+
+enum biot_field {
+  BIOT_VX = 0, BIOT_VY, BIOT_VZ, BIOT_QX, BIOT_QY, BIOT_QZ,
+  BIOT_TXX, BIOT_TYY, BIOT_TZZ, BIOT_TXY, BIOT_TXZ, BIOT_TYZ, BIOT_P,
+  BIOT_NFIELD
+};
+#define BIOT_MASK(f) (1u << (f))
+/* The reference source: Ricker added to txx, tyy, tzz (kernel.cu:138-140). */
+#define BIOT_SRC_REFERENCE (BIOT_MASK(BIOT_TXX) | BIOT_MASK(BIOT_TYY) | BIOT_MASK(BIOT_TZZ))
+
+/* The 23 coefficient arrays of main.cpp:431-494, in this order. */
+enum biot_coef {
+  BIOT_W1X = 0, BIOT_W1Y, BIOT_W1Z, BIOT_W2X, BIOT_W2Y, BIOT_W2Z, BIOT_W3X, BIOT_W3Y, BIOT_W3Z,
+  BIOT_U1X, BIOT_U1Y, BIOT_U1Z, BIOT_U2X, BIOT_U2Y, BIOT_U2Z,
+  BIOT_PP1, BIOT_PP2, BIOT_SP1, BIOT_SP2, BIOT_SP3, BIOT_SPXY, BIOT_SPXZ, BIOT_SPYZ,
+  BIOT_NCOEF
+};
+
+typedef struct biot_grid {
+  int32_t nx, ny, nz;   /* cells per axis including the PML (NX_ext.., main.cpp:73-75) */
+  int32_t order;        /* space order: 2 (the reference stencil) or 4 */
+  int32_t precision;    /* enum biot_precision */
+  int32_t npml;         /* PML thickness in cells (NP, main.cpp:23) */
+  float dx;             /* grid spacing in m (H, main.cpp:88) */
+  float dt;             /* time step in s (DT, main.cpp:90) */
+  float pml_v0;         /* PML reference velocity (v0 = 4000, main.cpp:555) */
+  float pml_R;          /* PML reflection coefficient (RC = 1e-6, main.cpp:89) */
+  float Cv, Cp;         /* representation scales (Cv = 1e8, Cp = 1, main.cpp:248-249) */
+  int32_t device;       /* HIP device ordinal */
+} biot_grid;
+
+/* Per-cell material inputs, nx*ny*nz doubles each (x fastest).  The Biot moduli follow
+ * outputfunc.cpp:205-217: alpha = 1 - Kb/Ks, M = Kf Ks / (phi Ks + (alpha - phi) Kf),
+ * H = alpha^2 M + Kb + 4 mu / 3, lambda_c = H - 2 mu, C = alpha M, C1 = eta/perm,
+ * C2 = (1 + 2/porousm) tao rho_f / phi, rho = (1 - phi) rho_s + phi rho_f.
+ * A cell with mu == 0 and Kb == 0 is a fluid cell (outputfunc.cpp:176-193). */
+typedef struct biot_medium {
+  const double *Ks, *Kf, *Kb, *mu, *phi, *rhos, *rhof, *eta, *perm;
+  double tao;       /* 3.0 in the reference (outputfunc.cpp:213) */
+  double porousm;   /* 8.0 in the reference (outputfunc.cpp:164) */
+} biot_medium;
+
+typedef struct biot_handle biot_handle;
+
+/* ---- lifetime ---- */
+int biot_create(const biot_grid *grid, biot_handle **out);
+int biot_destroy(biot_handle *h);
+const char *biot_last_error(const biot_handle *h);
+const char *biot_status_string(int status);
+int biot_abi_version(void);
+
+/* ---- medium, coefficients, PML ---- */
+/* Moduli -> staggered averages -> the 23 coefficient arrays (main.cpp:322-494), then
+ * uploads them.  Replaces rank 0's create_model_all + averaging + per-rank coefficient
+ * loops (main.cpp:303-494). */
+int biot_set_medium(biot_handle *h, const biot_medium *m);
+/* Caller-built coefficient arrays in the reference representation (the 23 coefficient
+ * members of blockInfo, common.h:21-27). coef[BIOT_W3Z] may be NULL (never read). */
+int biot_set_coefficients(biot_handle *h, const float *const coef[BIOT_NCOEF]);
+/* Read back the coefficient arrays the handle holds (reference representation). */
+int biot_get_coefficients(biot_handle *h, float *const coef[BIOT_NCOEF]);
+/* The single-domain C-PML profile of one axis (main.cpp:537-566 rule): d and e=exp(-d dt)
+ * at integer (d, e) and half (d2, e2) nodes, before the x0.01 pack of main.cpp:1512. */
+int biot_get_pml_profile(biot_handle *h, int32_t axis, float *d, float *d2, float *e,
+                         float *e2);
+
+/* Host-only forms of the two set-up stages above (no device is touched): the 23
+ * coefficient arrays of main.cpp:431-494 from per-cell moduli (nx*ny*nz floats each), and
+ * the PML profile of one axis of length n. */
+int biot_build_coefficients(int32_t nx, int32_t ny, int32_t nz, float dt, float Cv, float Cp,
+                            const biot_medium *m, float *const coef[BIOT_NCOEF]);
+int biot_pml_profile_1d(int32_t n, float dx, float dt, int32_t npml, float v0, float R, float *d,
+                        float *d2, float *e, float *e2);
+
+/* ---- source, receivers ---- */
+/* Wavelet samples w[0..nt-1]; at step it, w[it] is added to each field of field_mask at
+ * linear cell `cell` before the velocity half step (kernel.cu:130-142, 2064-2071). */
+int biot_set_source(biot_handle *h, int64_t cell, uint32_t field_mask, const float *wavelet,
+                    int32_t nt);
+/* Reference Ricker (kernel.cu:1898): w[it] = -(1 - 2 pi^2 f^2 (t-T0)^2) exp(-pi^2 f^2 (t-T0)^2),
+ * t = it*dt, T0 = 1.2/f, evaluated in double and rounded to float. */
+int biot_ricker(float f0, float dt, int32_t nt, float *out);
+/* Receivers sample every field of field_mask at each cell after every step
+ * (kernel.cu:2197-2243).  Traces hold nt_max steps; layout [comp][it][rec], comp running
+ * over the set bits of field_mask in increasing field order. */
+int biot_set_receivers(biot_handle *h, const int64_t *cells, int64_t nrec, uint32_t field_mask,
+                       int32_t nt_max);
+
+/* ---- time loop ---- */
+int biot_reset(biot_handle *h);             /* zero wavefields and PML memory, step = 0 */
+int biot_run(biot_handle *h, int32_t nsteps); /* advance nsteps: source -> V -> T -> record */
+int biot_step_count(biot_handle *h, int32_t *it);
+int biot_synchronize(biot_handle *h);
+/* Wall time (ms, HIP events on the handle's stream) of the last biot_run. */
+int biot_last_run_ms(biot_handle *h, double *ms);
+/* Advance nsteps like biot_run, with a HIP event pair around every launch of the two
+ * stencil kernels on the handle's stream; returns their mean durations in ms. */
+int biot_profile_steps(biot_handle *h, int32_t nsteps, double *ms_velocity, double *ms_stress);
+
+/* ---- outputs ---- */
+int biot_read_traces(biot_handle *h, float *out, int64_t count);
+int biot_read_field(biot_handle *h, int32_t field, float *out);
+int biot_read_field_f64(biot_handle *h, int32_t field, double *out);
+int biot_write_field_f64(biot_handle *h, int32_t field, const double *in);
+/* xz slice at y index j of one field (the slice wfile3d writes, outputfunc.cpp:97-123):
+ * out[k*nx + i]. */
+int biot_snapshot_xz(biot_handle *h, int32_t field, int32_t j, float *out);
+
+/* ---- introspection ---- */
+/* FP16 storage scales: field value = stored * scale (reference representation). */
+int biot_get_field_scales(biot_handle *h, float *scales13);
+/* Device bytes held by the handle. */
+int biot_device_bytes(biot_handle *h, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

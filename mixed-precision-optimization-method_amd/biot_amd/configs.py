@@ -1,0 +1,73 @@
+"""The standard workloads of this solver, with their sizes.
+
+fwd64   : 64^3, dx 5 m, dt 0.4 ms, 500 steps, PML 10, homogeneous porous rock, 25 Hz
+          Ricker on txx/tyy/tzz/p at (32, 32, 32), 32 receivers on x = 16..47, y = 32,
+          z = 10 (the first plane inside the PML) recording vx, vz, p.
+layered256 : 256^3, 2000 steps, PML 20, three equal z-layers, 20 Hz Ricker at the centre,
+          256 x 256 receivers on the plane z = 20 recording vx, vz, p.
+perturbed512 : 512 x 512 x 384, 3000 steps, FP16 storage, the layered medium with +-10 %
+          Gaussian-correlated (50 m) perturbations of Kb, mu, phi; 128 x 128 receivers (every
+          4th cell) on z = 20.
+adjoint128 : 128^3, 1000 steps, checkpoints every 50 steps.
+batch16 : 16 shots in cubic windows of 96..192 cells cropped from perturbed512.
+"""
+import numpy as np
+
+from . import media
+
+DX = 5.0
+DT = 4e-4
+SRC_EXPLOSIVE = ("txx", "tyy", "tzz", "p")
+REC_FIELDS = ("vx", "vz", "p")
+
+
+def fwd64(order=4, nt=500):
+    n = 64
+    shape = (n, n, n)
+    rec = [(i, 32, 10) for i in range(16, 48)]
+    return dict(name="fwd64", nx=n, ny=n, nz=n, dx=DX, dt=DT, order=order, npml=10, nt=nt,
+                f0=25.0, src=(32, 32, 32), src_fields=SRC_EXPLOSIVE,
+                medium=media.homogeneous(shape), rec=rec, rec_fields=REC_FIELDS)
+
+
+def layered256(order=4, nt=2000, n=256):
+    shape = (n, n, n)
+    npml = 20
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    rec = np.stack([ii.ravel(), jj.ravel(), np.full(n * n, npml)], axis=1)
+    return dict(name="layered256", nx=n, ny=n, nz=n, dx=DX, dt=DT, order=order, npml=npml,
+                nt=nt, f0=20.0, src=(n // 2, n // 2, n // 2), src_fields=SRC_EXPLOSIVE,
+                medium=media.layered(shape), rec=rec, rec_fields=REC_FIELDS)
+
+
+def perturbed512(order=4, nt=3000, nx=512, ny=512, nz=384, seed=2024, stride=4):
+    shape = (nz, ny, nx)
+    npml = 20
+    ii, jj = np.meshgrid(np.arange(0, nx, stride), np.arange(0, ny, stride), indexing="xy")
+    rec = np.stack([ii.ravel(), jj.ravel(), np.full(ii.size, npml)], axis=1)
+    return dict(name="perturbed512", nx=nx, ny=ny, nz=nz, dx=DX, dt=DT, order=order,
+                npml=npml, nt=nt, f0=20.0, src=(nx // 2, ny // 2, nz // 2),
+                src_fields=SRC_EXPLOSIVE, medium=media.perturbed(shape, seed=seed), rec=rec,
+                rec_fields=REC_FIELDS)
+
+
+def cells(cfg, ijk):
+    ijk = np.asarray(ijk, dtype=np.int64).reshape(-1, 3)
+    return ijk[:, 0] + ijk[:, 1] * cfg["nx"] + ijk[:, 2] * cfg["nx"] * cfg["ny"]
+
+
+def make_solver(cfg, precision="fp32", order=None, nt=None, with_receivers=True):
+    """Builds a Solver for a workload dict (medium, source, receivers set)."""
+    from ._lib import ricker
+    from .solver import Solver
+
+    nt = cfg["nt"] if nt is None else nt
+    s = Solver(cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["dt"],
+               order=cfg["order"] if order is None else order, precision=precision,
+               npml=cfg["npml"])
+    w = ricker(cfg["f0"], cfg["dt"], nt)
+    s.set_source(int(cells(cfg, cfg["src"])[0]), w, cfg["src_fields"])
+    s.set_medium(cfg["medium"])
+    if with_receivers:
+        s.set_receivers(cells(cfg, cfg["rec"]), cfg["rec_fields"], nt)
+    return s

@@ -1,0 +1,613 @@
+// C-ABI of the Biot solver (include/biot.h): handle lifetime, device memory, set-up
+// uploads and the time loop.  Replaces the reference's callCUDAKernel (kernel.cu:1728)
+// and the caller-side allocation/copy of blockInfo (main.cpp:1376-1743).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/biot.h"
+#include "biot_device.hpp"
+#include "biot_handle.hpp"
+#include "biot_setup.hpp"
+
+using namespace biot;
+
+namespace {
+
+int fail(biot_handle *h, int code, const char *fmt, ...)
+{
+  if (h) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    h->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                       \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail((h), BIOT_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                                        \
+  } while (0)
+
+size_t elem_size(int prec) { return prec == BIOT_FP16 ? 2 : prec == BIOT_FP64 ? 8 : 4; }
+
+// host storage-type conversion
+void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst)
+{
+  if (prec == BIOT_FP32) {
+    float *d = (float *)dst;
+    for (int64_t c = 0; c < n; ++c) d[c] = (float)(src[c] * scale_inv);
+  } else if (prec == BIOT_FP16) {
+    _Float16 *d = (_Float16 *)dst;
+    for (int64_t c = 0; c < n; ++c) d[c] = (_Float16)(float)(src[c] * scale_inv);
+  } else {
+    double *d = (double *)dst;
+    for (int64_t c = 0; c < n; ++c) d[c] = src[c] * scale_inv;
+  }
+}
+void from_storage(int prec, const void *src, int64_t n, double scale, double *dst)
+{
+  if (prec == BIOT_FP32) {
+    const float *s = (const float *)src;
+    for (int64_t c = 0; c < n; ++c) dst[c] = (double)s[c] * scale;
+  } else if (prec == BIOT_FP16) {
+    const _Float16 *s = (const _Float16 *)src;
+    for (int64_t c = 0; c < n; ++c) dst[c] = (double)(float)s[c] * scale;
+  } else {
+    const double *s = (const double *)src;
+    for (int64_t c = 0; c < n; ++c) dst[c] = s[c] * scale;
+  }
+}
+
+template <typename S>
+void fill_args(biot_handle *h, StepArgs<S> &a)
+{
+  std::memset(&a, 0, sizeof a);
+  for (int f = 0; f < NFIELD; ++f) a.f[f] = (S *)h->field_ptr(f);
+  for (int k = 0; k < NKC; ++k) {
+    a.k[k] = (const S *)((char *)h->d_coef + (size_t)k * h->n * h->esz);
+    a.ks[k] = h->ks[k];
+  }
+  for (int ax = 0; ax < 3; ++ax) {
+    a.carry[ax] = (S *)h->d_carry[ax];
+    a.slab[ax] = h->slab[ax];
+    a.L[ax] = h->L[ax];
+    a.R[ax] = h->R[ax];
+    for (int q = 0; q < 4; ++q) a.prof[ax][q] = h->d_prof[ax][q];
+  }
+  a.nx = h->g.nx;
+  a.ny = h->g.ny;
+  a.nz = h->g.nz;
+  a.sy = h->g.nx;
+  a.sz = (int64_t)h->g.nx * h->g.ny;
+}
+
+}  // namespace
+
+void *biot_handle::field_ptr(int f) const { return (char *)d_fields + (size_t)f * n * esz; }
+
+// ------------------------------------------------------------------------------------
+// lifetime
+// ------------------------------------------------------------------------------------
+extern "C" int biot_abi_version(void) { return BIOT_ABI_VERSION; }
+
+extern "C" const char *biot_status_string(int s)
+{
+  switch (s) {
+    case BIOT_OK: return "ok";
+    case BIOT_ERR_ARG: return "invalid argument";
+    case BIOT_ERR_HIP: return "HIP runtime error";
+    case BIOT_ERR_STATE: return "call out of order";
+    case BIOT_ERR_NONFINITE: return "non-finite value detected";
+    case BIOT_ERR_RANGE: return "value outside the storage range";
+    case BIOT_ERR_NOMEM: return "out of memory";
+    default: return "unknown status";
+  }
+}
+
+extern "C" const char *biot_last_error(const biot_handle *h)
+{
+  return h ? h->err.c_str() : "null handle";
+}
+
+static int validate_grid(const biot_grid *g, std::string &why)
+{
+  char buf[256];
+  if (g->order != 2 && g->order != 4) {
+    snprintf(buf, sizeof buf, "order must be 2 or 4, got %d", g->order);
+    why = buf;
+    return BIOT_ERR_ARG;
+  }
+  const int h = g->order / 2;
+  if (g->nx < 2 * h + 1 || g->ny < 2 * h + 1 || g->nz < 2 * h + 1) {
+    snprintf(buf, sizeof buf, "grid %dx%dx%d too small for order %d", g->nx, g->ny, g->nz,
+             g->order);
+    why = buf;
+    return BIOT_ERR_ARG;
+  }
+  if ((int64_t)g->nx * g->ny * g->nz > (int64_t)1 << 34) {
+    why = "grid too large";
+    return BIOT_ERR_ARG;
+  }
+  if (g->precision < 0 || g->precision > 2) {
+    why = "precision must be BIOT_FP32, BIOT_FP16 or BIOT_FP64";
+    return BIOT_ERR_ARG;
+  }
+  if (!(g->dx > 0) || !(g->dt > 0) || !(g->Cv > 0) || !(g->Cp > 0)) {
+    why = "dx, dt, Cv and Cp must be positive";
+    return BIOT_ERR_ARG;
+  }
+  if (g->npml < 0 || 2 * g->npml + 2 > std::min({g->nx, g->ny, g->nz})) {
+    why = "npml out of range for the grid";
+    return BIOT_ERR_ARG;
+  }
+  if (g->npml > 0 && (!(g->pml_v0 > 0) || !(g->pml_R > 0) || !(g->pml_R < 1))) {
+    why = "pml_v0 must be > 0 and 0 < pml_R < 1";
+    return BIOT_ERR_ARG;
+  }
+  return BIOT_OK;
+}
+
+extern "C" int biot_create(const biot_grid *g, biot_handle **out)
+{
+  if (!g || !out) return BIOT_ERR_ARG;
+  *out = nullptr;
+  std::string why;
+  int rc = validate_grid(g, why);
+  if (rc) return rc;
+  biot_handle *h = new biot_handle();
+  h->g = *g;
+  h->prec = g->precision;
+  h->esz = elem_size(g->precision);
+  h->n = (int64_t)g->nx * g->ny * g->nz;
+  for (int k = 0; k < NKC; ++k) h->ks[k] = 1.0f;
+  for (int f = 0; f < NFIELD; ++f) h->fscale[f] = 1.0f;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    delete h;
+    return BIOT_ERR_HIP;
+  }
+  if (g->device < 0 || g->device >= ndev) {
+    delete h;
+    return BIOT_ERR_ARG;
+  }
+  if (hipSetDevice(g->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess ||
+      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+    delete h;
+    return BIOT_ERR_HIP;
+  }
+  h->shape = default_shape(h->prec, g->order);
+  // PML profiles (main.cpp:537-566) and slab widths
+  const int dims[3] = {g->nx, g->ny, g->nz};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int n = dims[ax];
+    auto &P = h->prof[ax];
+    for (auto &v : P) v.assign(n, 0.0f);
+    if (g->npml > 0)
+      build_pml_profile(n, g->dx, g->dt, g->npml, g->pml_v0, g->pml_R, P[0].data(), P[1].data(),
+                        P[2].data(), P[3].data());
+    int L = 0, R = 0;
+    for (int i = 0; i < n; ++i) {
+      const bool nz_ = P[0][i] != 0.0f || P[1][i] != 0.0f;
+      if (!nz_) continue;
+      if (i < n / 2) L = std::max(L, i + 1);
+      else R = std::max(R, n - i);
+    }
+    h->L[ax] = L;
+    h->R[ax] = R;
+  }
+  h->slab[0] = (int64_t)(h->L[0] + h->R[0]) * g->ny * g->nz;
+  h->slab[1] = (int64_t)g->nx * (h->L[1] + h->R[1]) * g->nz;
+  h->slab[2] = (int64_t)g->nx * g->ny * (h->L[2] + h->R[2]);
+  rc = h->allocate();
+  if (rc) {
+    std::string e = h->err;
+    delete h;
+    return rc;
+  }
+  rc = h->upload_profiles();
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return BIOT_OK;
+}
+
+extern "C" int biot_destroy(biot_handle *h)
+{
+  if (!h) return BIOT_OK;
+  delete h;
+  return BIOT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// medium / coefficients
+// ------------------------------------------------------------------------------------
+extern "C" int biot_set_medium(biot_handle *h, const biot_medium *m)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (!m || !m->Ks || !m->Kf || !m->Kb || !m->mu || !m->phi || !m->rhos || !m->rhof || !m->eta ||
+      !m->perm)
+    return fail(h, BIOT_ERR_ARG, "biot_set_medium: every per-cell array must be given");
+  for (int64_t c = 0; c < h->n; ++c) {
+    const bool fluid = m->mu[c] == 0.0 && m->Kb[c] == 0.0;
+    if (!(m->rhos[c] > 0) || !(m->rhof[c] > 0) || !(m->Kf[c] > 0))
+      return fail(h, BIOT_ERR_ARG, "cell %lld: rho_s, rho_f and Kf must be > 0", (long long)c);
+    if (!fluid && (!(m->phi[c] > 0) || !(m->phi[c] <= 1) || !(m->Ks[c] > 0) || !(m->perm[c] > 0) ||
+                   !(m->eta[c] >= 0) || !(m->mu[c] >= 0) || !(m->Kb[c] >= 0)))
+      return fail(h, BIOT_ERR_ARG, "cell %lld: invalid porous-medium parameters", (long long)c);
+  }
+  Moduli md;
+  Averages av;
+  build_moduli(h->n, *m, md);
+  build_averages(h->g.nx, h->g.ny, h->g.nz, md, av);
+  build_coefficients(h->g.nx, h->g.ny, h->g.nz, h->g.dt, h->g.Cv, h->g.Cp, md, av, h->coef_ref);
+  return h->upload_coefficients();
+}
+
+extern "C" int biot_build_coefficients(int32_t nx, int32_t ny, int32_t nz, float dt, float Cv,
+                                       float Cp, const biot_medium *m, float *const coef[BIOT_NCOEF])
+{
+  if (!m || !coef || nx < 3 || ny < 3 || nz < 3) return BIOT_ERR_ARG;
+  for (int a = 0; a < BIOT_NCOEF; ++a)
+    if (!coef[a]) return BIOT_ERR_ARG;
+  const int64_t n = (int64_t)nx * ny * nz;
+  Moduli md;
+  Averages av;
+  std::vector<float> K[BIOT_NCOEF];
+  build_moduli(n, *m, md);
+  build_averages(nx, ny, nz, md, av);
+  build_coefficients(nx, ny, nz, dt, Cv, Cp, md, av, K);
+  for (int a = 0; a < BIOT_NCOEF; ++a) std::memcpy(coef[a], K[a].data(), (size_t)n * sizeof(float));
+  return BIOT_OK;
+}
+
+extern "C" int biot_pml_profile_1d(int32_t n, float dx, float dt, int32_t npml, float v0, float R,
+                                   float *d, float *d2, float *e, float *e2)
+{
+  if (n <= 0 || npml <= 0 || !(dx > 0) || !d || !d2 || !e || !e2) return BIOT_ERR_ARG;
+  build_pml_profile(n, dx, dt, npml, v0, R, d, d2, e, e2);
+  return BIOT_OK;
+}
+
+extern "C" int biot_set_coefficients(biot_handle *h, const float *const coef[BIOT_NCOEF])
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (!coef) return fail(h, BIOT_ERR_ARG, "coef is null");
+  for (int a = 0; a < BIOT_NCOEF; ++a) {
+    if (!coef[a] && a != BIOT_W3Z) return fail(h, BIOT_ERR_ARG, "coefficient array %d is null", a);
+    if (coef[a])
+      h->coef_ref[a].assign(coef[a], coef[a] + h->n);
+    else
+      h->coef_ref[a].assign((size_t)h->n, 0.0f);
+  }
+  return h->upload_coefficients();
+}
+
+extern "C" int biot_get_coefficients(biot_handle *h, float *const coef[BIOT_NCOEF])
+{
+  if (!h || !coef) return BIOT_ERR_ARG;
+  if (!h->have_coef) return fail(h, BIOT_ERR_STATE, "no coefficients set");
+  for (int a = 0; a < BIOT_NCOEF; ++a)
+    if (coef[a]) std::memcpy(coef[a], h->coef_ref[a].data(), (size_t)h->n * sizeof(float));
+  return BIOT_OK;
+}
+
+extern "C" int biot_get_pml_profile(biot_handle *h, int32_t axis, float *d, float *d2, float *e,
+                                    float *e2)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (axis < 0 || axis > 2) return fail(h, BIOT_ERR_ARG, "axis must be 0, 1 or 2");
+  const auto &P = h->prof[axis];
+  const size_t n = P[0].size();
+  if (d) std::memcpy(d, P[0].data(), n * 4);
+  if (d2) std::memcpy(d2, P[1].data(), n * 4);
+  if (e) std::memcpy(e, P[2].data(), n * 4);
+  if (e2) std::memcpy(e2, P[3].data(), n * 4);
+  return BIOT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// source / receivers
+// ------------------------------------------------------------------------------------
+extern "C" int biot_ricker(float f0, float dt, int32_t nt, float *out)
+{
+  if (!out || nt < 0 || !(f0 > 0)) return BIOT_ERR_ARG;
+  for (int it = 0; it < nt; ++it) out[it] = ricker(it, dt, f0);
+  return BIOT_OK;
+}
+
+extern "C" int biot_set_source(biot_handle *h, int64_t cell, uint32_t mask, const float *w,
+                               int32_t nt)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (cell < 0 || cell >= h->n) return fail(h, BIOT_ERR_ARG, "source cell %lld outside grid", (long long)cell);
+  if (mask >> NFIELD) return fail(h, BIOT_ERR_ARG, "source field mask has bits above field 12");
+  if (nt < 0 || (nt > 0 && !w)) return fail(h, BIOT_ERR_ARG, "wavelet null or nt < 0");
+  h->src_cell = cell;
+  h->src_mask = mask;
+  h->wavelet.assign(w, w + nt);
+  if (h->d_wavelet) (void)hipFree(h->d_wavelet);
+  h->d_wavelet = nullptr;
+  if (nt > 0) {
+    HIPCHK(h, hipMalloc(&h->d_wavelet, (size_t)nt * sizeof(float)));
+    HIPCHK(h, hipMemcpy(h->d_wavelet, w, (size_t)nt * sizeof(float), hipMemcpyHostToDevice));
+  }
+  if (h->prec == BIOT_FP16 && h->have_coef) return h->upload_coefficients();  // rescale
+  return BIOT_OK;
+}
+
+extern "C" int biot_set_receivers(biot_handle *h, const int64_t *cells, int64_t nrec,
+                                  uint32_t mask, int32_t nt_max)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (nrec < 0 || (nrec > 0 && !cells)) return fail(h, BIOT_ERR_ARG, "receiver list invalid");
+  if (mask >> NFIELD) return fail(h, BIOT_ERR_ARG, "receiver field mask has bits above field 12");
+  if (nt_max < 0) return fail(h, BIOT_ERR_ARG, "nt_max < 0");
+  for (int64_t r = 0; r < nrec; ++r)
+    if (cells[r] < 0 || cells[r] >= h->n)
+      return fail(h, BIOT_ERR_ARG, "receiver %lld at cell %lld outside grid", (long long)r,
+                  (long long)cells[r]);
+  if (h->d_rec) (void)hipFree(h->d_rec);
+  if (h->d_traces) (void)hipFree(h->d_traces);
+  h->d_rec = nullptr;
+  h->d_traces = nullptr;
+  h->ncomp = 0;
+  for (int f = 0; f < NFIELD; ++f)
+    if (mask & (1u << f)) h->comp[h->ncomp++] = f;
+  h->nrec = nrec;
+  h->rec_mask = mask;
+  h->nt_max = nt_max;
+  const size_t tr = (size_t)h->ncomp * nt_max * nrec;
+  if (nrec > 0) {
+    HIPCHK(h, hipMalloc(&h->d_rec, (size_t)nrec * sizeof(int64_t)));
+    HIPCHK(h, hipMemcpy(h->d_rec, cells, (size_t)nrec * sizeof(int64_t), hipMemcpyHostToDevice));
+  }
+  if (tr > 0) {
+    HIPCHK(h, hipMalloc(&h->d_traces, tr * sizeof(float)));
+    HIPCHK(h, hipMemsetAsync(h->d_traces, 0, tr * sizeof(float), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return BIOT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// time loop
+// ------------------------------------------------------------------------------------
+extern "C" int biot_reset(biot_handle *h)
+{
+  if (!h) return BIOT_ERR_ARG;
+  HIPCHK(h, hipMemsetAsync(h->d_fields, 0, (size_t)NFIELD * h->n * h->esz, h->stream));
+  for (int ax = 0; ax < 3; ++ax)
+    if (h->slab[ax] > 0)
+      HIPCHK(h, hipMemsetAsync(h->d_carry[ax], 0, (size_t)NCARRY * h->slab[ax] * h->esz, h->stream));
+  if (h->d_traces)
+    HIPCHK(h, hipMemsetAsync(h->d_traces, 0, (size_t)h->ncomp * h->nt_max * h->nrec * sizeof(float),
+                             h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->it = 0;
+  return BIOT_OK;
+}
+
+int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks)
+{
+  SourceArgs sa;
+  std::memset(&sa, 0, sizeof sa);
+  RecordArgs ra;
+  std::memset(&ra, 0, sizeof ra);
+  for (int f = 0; f < NFIELD; ++f) {
+    sa.f[f] = field_ptr(f);
+    ra.f[f] = field_ptr(f);
+    sa.inv_scale[f] = 1.0f / fscale[f];
+  }
+  sa.cell = src_cell;
+  sa.mask = src_mask;
+  sa.wavelet = (const float *)d_wavelet;
+  ra.rec = (const int64_t *)d_rec;
+  ra.nrec = nrec;
+  ra.ncomp = ncomp;
+  for (int c = 0; c < ncomp; ++c) {
+    ra.comp[c] = comp[c];
+    ra.scale[c] = fscale[comp[c]];
+  }
+  ra.traces = (float *)d_traces;
+  ra.nt_max = nt_max;
+
+  alignas(16) unsigned char args[sizeof(StepArgs<double>)];
+  if (prec == BIOT_FP32) fill_args(this, *(StepArgs<float> *)args);
+  else if (prec == BIOT_FP16) fill_args(this, *(StepArgs<_Float16> *)args);
+  else fill_args(this, *(StepArgs<double> *)args);
+  // zchunk lives at the same offset for every storage type
+  if (prec == BIOT_FP32) ((StepArgs<float> *)args)->zchunk = shape.zchunk;
+  else if (prec == BIOT_FP16) ((StepArgs<_Float16> *)args)->zchunk = shape.zchunk;
+  else ((StepArgs<double> *)args)->zchunk = shape.zchunk;
+
+  for (int s = 0; s < nsteps; ++s) {
+    if (d_wavelet && src_mask && it < (int)wavelet.size()) {
+      sa.it = it;
+      hipError_t e = launch_source(prec, sa, stream);
+      if (e != hipSuccess) return fail(this, BIOT_ERR_HIP, "source launch: %s", hipGetErrorString(e));
+    }
+    if (marks) (void)hipEventRecord(marks[3 * s], stream);
+    hipError_t e = launch_velocity(prec, g.order, args, shape, stream);
+    if (e != hipSuccess) return fail(this, BIOT_ERR_HIP, "velocity launch: %s", hipGetErrorString(e));
+    if (marks) (void)hipEventRecord(marks[3 * s + 1], stream);
+    e = launch_stress(prec, g.order, args, shape, stream);
+    if (e != hipSuccess) return fail(this, BIOT_ERR_HIP, "stress launch: %s", hipGetErrorString(e));
+    if (marks) (void)hipEventRecord(marks[3 * s + 2], stream);
+    if (d_traces && it < nt_max) {
+      ra.it = it;
+      e = launch_record(prec, ra, stream);
+      if (e != hipSuccess) return fail(this, BIOT_ERR_HIP, "record launch: %s", hipGetErrorString(e));
+    }
+    ++it;
+  }
+  return BIOT_OK;
+}
+
+extern "C" int biot_run(biot_handle *h, int32_t nsteps)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (nsteps < 0) return fail(h, BIOT_ERR_ARG, "nsteps < 0");
+  if (!h->have_coef) return fail(h, BIOT_ERR_STATE, "biot_run before biot_set_medium/coefficients");
+  HIPCHK(h, hipSetDevice(h->g.device));
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  int rc = h->enqueue_steps(nsteps, nullptr);
+  if (rc) {
+    (void)hipStreamSynchronize(h->stream);
+    return rc;
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  return BIOT_OK;
+}
+
+extern "C" int biot_profile_steps(biot_handle *h, int32_t nsteps, double *ms_v, double *ms_t)
+{
+  if (!h || !ms_v || !ms_t) return BIOT_ERR_ARG;
+  if (nsteps <= 0 || nsteps > 100000) return fail(h, BIOT_ERR_ARG, "nsteps must be in 1..100000");
+  if (!h->have_coef) return fail(h, BIOT_ERR_STATE, "biot_profile_steps before set_medium");
+  HIPCHK(h, hipSetDevice(h->g.device));
+  std::vector<hipEvent_t> ev((size_t)3 * nsteps, nullptr);
+  int rc = BIOT_OK;
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = BIOT_ERR_HIP;
+  if (!rc) rc = h->enqueue_steps(nsteps, ev.data());
+  double sv = 0, stt = 0;
+  if (!rc && hipStreamSynchronize(h->stream) == hipSuccess) {
+    for (int s = 0; s < nsteps; ++s) {
+      float a = 0, b = 0;
+      (void)hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]);
+      (void)hipEventElapsedTime(&b, ev[3 * s + 1], ev[3 * s + 2]);
+      sv += a;
+      stt += b;
+    }
+  } else if (!rc) {
+    rc = fail(h, BIOT_ERR_HIP, "stream synchronize failed while profiling");
+  }
+  (void)hipStreamSynchronize(h->stream);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  *ms_v = sv / nsteps;
+  *ms_t = stt / nsteps;
+  return BIOT_OK;
+}
+
+extern "C" int biot_step_count(biot_handle *h, int32_t *it)
+{
+  if (!h || !it) return BIOT_ERR_ARG;
+  *it = h->it;
+  return BIOT_OK;
+}
+
+extern "C" int biot_synchronize(biot_handle *h)
+{
+  if (!h) return BIOT_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return BIOT_OK;
+}
+
+extern "C" int biot_last_run_ms(biot_handle *h, double *ms)
+{
+  if (!h || !ms) return BIOT_ERR_ARG;
+  *ms = h->last_ms;
+  return BIOT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// outputs
+// ------------------------------------------------------------------------------------
+extern "C" int biot_read_traces(biot_handle *h, float *out, int64_t count)
+{
+  if (!h || !out) return BIOT_ERR_ARG;
+  const int64_t need = (int64_t)h->ncomp * h->nt_max * h->nrec;
+  if (count < need)
+    return fail(h, BIOT_ERR_ARG, "trace buffer holds %lld, need %lld", (long long)count,
+                (long long)need);
+  if (need == 0) return BIOT_OK;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(out, h->d_traces, (size_t)need * sizeof(float), hipMemcpyDeviceToHost));
+  return BIOT_OK;
+}
+
+extern "C" int biot_read_field_f64(biot_handle *h, int32_t field, double *out)
+{
+  if (!h || !out) return BIOT_ERR_ARG;
+  if (field < 0 || field >= NFIELD) return fail(h, BIOT_ERR_ARG, "field %d out of range", field);
+  std::vector<unsigned char> tmp((size_t)h->n * h->esz);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(tmp.data(), h->field_ptr(field), tmp.size(), hipMemcpyDeviceToHost));
+  from_storage(h->prec, tmp.data(), h->n, h->fscale[field], out);
+  return BIOT_OK;
+}
+
+extern "C" int biot_read_field(biot_handle *h, int32_t field, float *out)
+{
+  if (!h || !out) return BIOT_ERR_ARG;
+  std::vector<double> tmp((size_t)h->n);
+  int rc = biot_read_field_f64(h, field, tmp.data());
+  if (rc) return rc;
+  for (int64_t c = 0; c < h->n; ++c) out[c] = (float)tmp[c];
+  return BIOT_OK;
+}
+
+extern "C" int biot_write_field_f64(biot_handle *h, int32_t field, const double *in)
+{
+  if (!h || !in) return BIOT_ERR_ARG;
+  if (field < 0 || field >= NFIELD) return fail(h, BIOT_ERR_ARG, "field %d out of range", field);
+  std::vector<unsigned char> tmp((size_t)h->n * h->esz);
+  to_storage(h->prec, in, h->n, 1.0 / h->fscale[field], tmp.data());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(h->field_ptr(field), tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+  return BIOT_OK;
+}
+
+extern "C" int biot_snapshot_xz(biot_handle *h, int32_t field, int32_t j, float *out)
+{
+  if (!h || !out) return BIOT_ERR_ARG;
+  if (field < 0 || field >= NFIELD) return fail(h, BIOT_ERR_ARG, "field %d out of range", field);
+  if (j < 0 || j >= h->g.ny) return fail(h, BIOT_ERR_ARG, "y index %d outside grid", j);
+  const int nx = h->g.nx, nz = h->g.nz;
+  std::vector<unsigned char> row((size_t)nx * nz * h->esz);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  // one 2-D copy: nz rows of nx elements, source pitch nx*ny elements
+  HIPCHK(h, hipMemcpy2D(row.data(), (size_t)nx * h->esz,
+                        (char *)h->field_ptr(field) + (size_t)j * nx * h->esz,
+                        (size_t)nx * h->g.ny * h->esz, (size_t)nx * h->esz, (size_t)nz,
+                        hipMemcpyDeviceToHost));
+  std::vector<double> tmp((size_t)nx * nz);
+  from_storage(h->prec, row.data(), (int64_t)nx * nz, h->fscale[field], tmp.data());
+  for (size_t c = 0; c < tmp.size(); ++c) out[c] = (float)tmp[c];
+  return BIOT_OK;
+}
+
+extern "C" int biot_get_field_scales(biot_handle *h, float *scales13)
+{
+  if (!h || !scales13) return BIOT_ERR_ARG;
+  for (int f = 0; f < NFIELD; ++f) scales13[f] = h->fscale[f];
+  return BIOT_OK;
+}
+
+extern "C" int biot_device_bytes(biot_handle *h, int64_t *bytes)
+{
+  if (!h || !bytes) return BIOT_ERR_ARG;
+  *bytes = h->device_bytes;
+  return BIOT_OK;
+}

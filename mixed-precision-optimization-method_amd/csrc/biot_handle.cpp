@@ -1,0 +1,225 @@
+// Device memory management and coefficient/profile uploads of a biot_handle.
+#include "biot_handle.hpp"
+#include "biot_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace biot;
+
+biot_handle::~biot_handle()
+{
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (d_fields) (void)hipFree(d_fields);
+  if (d_coef) (void)hipFree(d_coef);
+  for (auto *p : d_carry)
+    if (p) (void)hipFree(p);
+  for (auto &ax : d_prof)
+    for (auto *p : ax)
+      if (p) (void)hipFree(p);
+  if (d_wavelet) (void)hipFree(d_wavelet);
+  if (d_rec) (void)hipFree(d_rec);
+  if (d_traces) (void)hipFree(d_traces);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+LaunchShape default_shape(int prec, int order)
+{
+  (void)prec;
+  (void)order;
+  return LaunchShape{64, 4, 16};
+}
+
+int biot_handle::allocate()
+{
+  auto alloc = [&](void **p, size_t bytes) -> bool {
+    if (bytes == 0) return true;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      *p = nullptr;
+      return false;
+    }
+    device_bytes += (int64_t)bytes;
+    return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
+  };
+  bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, (size_t)NKC * n * esz);
+  for (int ax = 0; ax < 3 && ok; ++ax) ok = alloc(&d_carry[ax], (size_t)NCARRY * slab[ax] * esz);
+  const int dims[3] = {g.nx, g.ny, g.nz};
+  const size_t asz = prec == BIOT_FP64 ? 8 : 4;
+  for (int ax = 0; ax < 3 && ok; ++ax)
+    for (int q = 0; q < 4 && ok; ++q) ok = alloc(&d_prof[ax][q], (size_t)dims[ax] * asz);
+  if (!ok) {
+    err = "device allocation failed";
+    return BIOT_ERR_NOMEM;
+  }
+  if (hipStreamSynchronize(stream) != hipSuccess) {
+    err = "hipStreamSynchronize failed after allocation";
+    return BIOT_ERR_HIP;
+  }
+  return BIOT_OK;
+}
+
+// e and a = -dt*d/2 as the kernels use them.  The reference stores e and d multiplied by
+// Cpml = 0.01f (main.cpp:1512-1523) and the kernels multiply back by 100.0 in double
+// (kernel.cu:188): the effective values are reproduced here.
+int biot_handle::upload_profiles()
+{
+  const int dims[3] = {g.nx, g.ny, g.nz};
+  const float Cpml = 0.01f;
+  for (int ax = 0; ax < 3; ++ax) {
+    const int n_ = dims[ax];
+    std::vector<double> ei(n_), ai(n_), eh(n_), ah(n_);
+    for (int i = 0; i < n_; ++i) {
+      const float d_s = prof[ax][0][i] * Cpml, d2_s = prof[ax][1][i] * Cpml;
+      const float e_s = prof[ax][2][i] * Cpml, e2_s = prof[ax][3][i] * Cpml;
+      ei[i] = (double)e_s * 100.0;
+      eh[i] = (double)e2_s * 100.0;
+      ai[i] = (double)(float)(-g.dt * d_s) * 100.0 * 0.5;
+      ah[i] = (double)(float)(-g.dt * d2_s) * 100.0 * 0.5;
+    }
+    const std::vector<double> *src[4] = {&ei, &ai, &eh, &ah};
+    for (int q = 0; q < 4; ++q) {
+      if (prec == BIOT_FP64) {
+        if (hipMemcpy(d_prof[ax][q], src[q]->data(), n_ * 8, hipMemcpyHostToDevice) != hipSuccess)
+          return BIOT_ERR_HIP;
+      } else {
+        std::vector<float> f(n_);
+        for (int i = 0; i < n_; ++i) f[i] = (float)(*src[q])[i];
+        if (hipMemcpy(d_prof[ax][q], f.data(), n_ * 4, hipMemcpyHostToDevice) != hipSuccess)
+          return BIOT_ERR_HIP;
+      }
+    }
+  }
+  return BIOT_OK;
+}
+
+namespace {
+struct KernCtx {
+  const std::vector<float> *ref;
+  double Hinv;
+  std::vector<double> *K;  // NKC arrays
+};
+// reference arrays -> kernel coefficients: 1/dx folded into every coefficient that
+// multiplies a difference, the kernels' 0.01 (d_Cpp*, d_Csp*, kernel.cu:26-31) folded into
+// the pressure/stress ones.
+void kern_body(int64_t lo, int64_t hi, void *p)
+{
+  auto *cx = static_cast<KernCtx *>(p);
+  const std::vector<float> *R = cx->ref;
+  std::vector<double> *K = cx->K;
+  const double H = cx->Hinv;
+  for (int64_t c = lo; c < hi; ++c) {
+    for (int d = 0; d < 3; ++d) {
+      K[K_U1X + d][c] = (double)R[BIOT_U1X + d][c] * H;
+      K[K_U2X + d][c] = (double)R[BIOT_U2X + d][c];
+      K[K_W1X + d][c] = (double)R[BIOT_W1X + d][c];
+      K[K_W2X + d][c] = (double)R[BIOT_W2X + d][c] * H;
+    }
+    K[K_W3X][c] = (double)R[BIOT_W3X][c] * H;
+    K[K_W3Y][c] = (double)R[BIOT_W3Y][c] * H;
+    K[K_P1][c] = 0.01 * R[BIOT_PP1][c] * H;
+    K[K_P2][c] = 0.01 * R[BIOT_PP2][c] * H;
+    K[K_S1][c] = 0.01 * R[BIOT_SP1][c] * H;
+    K[K_S2][c] = 0.01 * R[BIOT_SP2][c] * H;
+    K[K_S3][c] = 0.01 * R[BIOT_SP3][c] * H;
+    K[K_SXY][c] = 0.01 * R[BIOT_SPXY][c] * H;
+    K[K_SXZ][c] = 0.01 * R[BIOT_SPXZ][c] * H;
+    K[K_SYZ][c] = 0.01 * R[BIOT_SPYZ][c] * H;
+  }
+}
+double amax(const std::vector<double> &v)
+{
+  double m = 0;
+  for (double x : v) m = std::max(m, std::fabs(x));
+  return m;
+}
+}  // namespace
+
+// Builds the 22 kernel coefficient arrays from the reference arrays and uploads them.
+// FP16 storage: each field group is stored divided by a scale (stresses/pressure by the
+// peak source amplitude A; velocities by A*sqrt(max U1 / max S2), which makes the
+// velocity and stress update coefficients both equal to the CFL number c*dt/dx; the
+// relative fluid velocity by its forced response A * max((|W2|+|W3|)/(1-W1)) * 0.2, the
+// 0.2 being k*dx of a wave sampled at ~30 points per wavelength); every coefficient array
+// is stored divided by its largest magnitude so it lies in [-1, 1].
+int biot_handle::upload_coefficients()
+{
+  std::vector<double> K[NKC];
+  for (auto &v : K) v.assign((size_t)n, 0.0);
+  KernCtx cx{coef_ref, 1.0 / (double)g.dx, K};
+  parallel_for(n, kern_body, &cx);
+
+  for (int f = 0; f < NFIELD; ++f) fscale[f] = 1.0f;
+  for (int k = 0; k < NKC; ++k) ks[k] = 1.0f;
+  if (prec == BIOT_FP16) {
+    double A = 0;
+    for (float w : wavelet) A = std::max(A, (double)std::fabs(w));
+    if (!(A > 0)) A = 1.0;
+    const double u1 = std::max({amax(K[K_U1X]), amax(K[K_U1Y]), amax(K[K_U1Z])});
+    const double s2 = amax(K[K_S2]);
+    const double sig_t = A, sig_p = A;
+    const double sig_v = (u1 > 0 && s2 > 0) ? A * std::sqrt(u1 / s2) : A;
+    double resp = 0;
+    for (int d = 0; d < 3; ++d) {
+      const auto &w1 = K[K_W1X + d], &w2 = K[K_W2X + d], &w3 = K[d == 2 ? K_W3Y : K_W3X + d];
+      for (int64_t c = 0; c < n; ++c) {
+        const double den = 1.0 - w1[c];
+        if (den > 0) resp = std::max(resp, (std::fabs(w2[c]) + std::fabs(w3[c])) / den);
+      }
+    }
+    const double sig_q = resp > 0 ? A * resp * 0.2 : sig_v;
+    const double sv = sig_v, sq = sig_q, stau = sig_t, sp = sig_p;
+    auto scale_arr = [&](int k, double f) {
+      for (auto &x : K[k]) x *= f;
+    };
+    for (int d = 0; d < 3; ++d) {
+      scale_arr(K_U1X + d, stau / sv);
+      scale_arr(K_U2X + d, sq / sv);
+      scale_arr(K_W2X + d, stau / sq);
+    }
+    scale_arr(K_W3X, sp / sq);
+    scale_arr(K_W3Y, sp / sq);
+    scale_arr(K_P1, sv / sp);
+    scale_arr(K_P2, sq / sp);
+    scale_arr(K_S1, sv / stau);
+    scale_arr(K_S2, sv / stau);
+    scale_arr(K_S3, sq / stau);
+    scale_arr(K_SXY, sv / stau);
+    scale_arr(K_SXZ, sv / stau);
+    scale_arr(K_SYZ, sv / stau);
+    for (int f = 0; f < 3; ++f) fscale[f] = (float)sv;
+    for (int f = 3; f < 6; ++f) fscale[f] = (float)sq;
+    for (int f = 6; f < 12; ++f) fscale[f] = (float)stau;
+    fscale[12] = (float)sp;
+    for (int k = 0; k < NKC; ++k) {
+      const double m = amax(K[k]);
+      if (m > 0) {
+        ks[k] = (float)m;
+        scale_arr(k, 1.0 / (double)ks[k]);
+      }
+    }
+  }
+
+  std::vector<unsigned char> buf((size_t)n * esz);
+  for (int k = 0; k < NKC; ++k) {
+    const std::vector<double> &src = K[k];
+    if (prec == BIOT_FP32) {
+      float *d = (float *)buf.data();
+      for (int64_t c = 0; c < n; ++c) d[c] = (float)src[c];
+    } else if (prec == BIOT_FP16) {
+      _Float16 *d = (_Float16 *)buf.data();
+      for (int64_t c = 0; c < n; ++c) d[c] = (_Float16)(float)src[c];
+    } else {
+      std::memcpy(buf.data(), src.data(), (size_t)n * 8);
+    }
+    if (hipMemcpy((char *)d_coef + (size_t)k * n * esz, buf.data(), (size_t)n * esz,
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      err = "coefficient upload failed";
+      return BIOT_ERR_HIP;
+    }
+  }
+  have_coef = true;
+  return BIOT_OK;
+}

@@ -1,0 +1,64 @@
+// The state behind a biot_handle: everything the reference kept in blockInfo
+// (common.h:6-133) plus the time-loop state of callCUDAKernel (kernel.cu:1728-1760).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/biot.h"
+#include "biot_device.hpp"
+
+struct biot_handle {
+  biot_grid g{};
+  int prec = 0;
+  size_t esz = 4;
+  int64_t n = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+
+  // device memory
+  void *d_fields = nullptr;          // 13 * n
+  void *d_coef = nullptr;            // 22 * n
+  void *d_carry[3] = {nullptr, nullptr, nullptr};
+  void *d_prof[3][4] = {};           // e_int, a_int, e_half, a_half per axis
+  int64_t slab[3] = {0, 0, 0};
+  int L[3] = {0, 0, 0}, R[3] = {0, 0, 0};
+  int64_t device_bytes = 0;
+
+  // host copies
+  std::vector<float> coef_ref[BIOT_NCOEF];  // reference representation
+  std::vector<float> prof[3][4];            // d, d2, e, e2 per axis (before the x0.01 pack)
+  bool have_coef = false;
+  float ks[biot::NKC];        // per-coefficient-array scale (binary16 storage)
+  float fscale[biot::NFIELD]; // per-field storage scale
+
+  // source
+  std::vector<float> wavelet;
+  void *d_wavelet = nullptr;
+  int64_t src_cell = 0;
+  uint32_t src_mask = 0;
+  // receivers
+  void *d_rec = nullptr;
+  void *d_traces = nullptr;
+  int64_t nrec = 0;
+  uint32_t rec_mask = 0;
+  int comp[biot::NFIELD] = {};
+  int ncomp = 0;
+  int nt_max = 0;
+
+  int it = 0;
+  double last_ms = 0.0;
+  biot::LaunchShape shape{64, 4, 16};
+
+  ~biot_handle();
+  void *field_ptr(int f) const;
+  int allocate();
+  int upload_profiles();
+  int upload_coefficients();
+  // marks: null, or 3 events per step recorded before velocity, between, after stress
+  int enqueue_steps(int nsteps, hipEvent_t *marks);
+};
+
+biot::LaunchShape default_shape(int prec, int order);

@@ -1,0 +1,168 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Bars: FP32 storage <= 1e-5 relative L2 on receiver traces (max over components) against the
+oracle's FP32 restatement of the reference arithmetic; FP64 <= 1e-10 against the oracle's
+FP64 mode; FP16 storage <= 1e-2 against the oracle's FP64 mode.  Source and receiver
+indexing are checked bit for bit at the full 256^3 size.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from biot_amd import BiotError, configs, media
+from biot_amd.solver import Solver
+from conftest import max_comp_rel_l2, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_traces(cfg, order, mode, nt):
+    o = oracle.OracleRun(cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["dt"], order=order,
+                         mode=mode, npml=cfg["npml"], medium=cfg["medium"])
+    w = oracle.ricker_series(nt, cfg["dt"], cfg["f0"])
+    tr = o.run(0, nt, nt, w, configs.cells(cfg, cfg["src"])[0], cfg["src_fields"],
+               configs.cells(cfg, cfg["rec"]), cfg["rec_fields"])
+    return tr, o
+
+
+def _gpu_traces(cfg, order, precision, nt):
+    s = configs.make_solver(cfg, precision=precision, order=order, nt=nt)
+    s.run(nt)
+    return s.traces(), s
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_fp32_fwd64_traces_match_oracle(order):
+    print(f"fwd64 fp32 order {order}", flush=True)
+    cfg = configs.fwd64(order=order)
+    nt = cfg["nt"]
+    ref, o = _oracle_traces(cfg, order, "f32", nt)
+    tr, s = _gpu_traces(cfg, order, "fp32", nt)
+    err = max_comp_rel_l2(tr, ref)
+    print(f"  traces rel L2 = {err:.3e}")
+    assert err <= 1e-5
+    for f in ("vx", "txx", "p", "qz", "tyz"):
+        assert rel_l2(s.field(f), o.field(f)) <= 1e-5, f
+    s.close()
+
+
+def test_fp64_fwd64_matches_oracle_f64():
+    print("fwd64 fp64 order 4", flush=True)
+    cfg = configs.fwd64(order=4)
+    nt = 300
+    ref, _ = _oracle_traces(cfg, 4, "f64", nt)
+    tr, s = _gpu_traces(cfg, 4, "fp64", nt)
+    err = max_comp_rel_l2(tr, ref)
+    print(f"  traces rel L2 = {err:.3e}")
+    assert err <= 1e-10
+    s.close()
+
+
+def test_fp16_fwd64_within_1e2_of_fp64_oracle():
+    print("fwd64 fp16 order 4", flush=True)
+    cfg = configs.fwd64(order=4)
+    nt = cfg["nt"]
+    ref, _ = _oracle_traces(cfg, 4, "f64", nt)
+    tr, s = _gpu_traces(cfg, 4, "fp16", nt)
+    err = max_comp_rel_l2(tr, ref)
+    print(f"  fp16 traces rel L2 vs fp64 = {err:.3e}; scales {s.field_scales()}")
+    assert np.isfinite(tr).all()
+    assert err <= 1e-2
+    s.close()
+
+
+def _random_state(shape, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal(shape).astype(np.float32).astype(np.float64) for _ in range(13)]
+
+
+@pytest.mark.parametrize("precision,mode,tol", [("fp32", "f32", 1e-5), ("fp64", "f64", 1e-11)])
+def test_full_size_random_state_matches_oracle(precision, mode, tol):
+    """256^3, order 4, layered medium, PML 20: every field everywhere (PML slabs and edges
+    included) after 4 steps from a random state."""
+    print(f"layered256 random state {precision}", flush=True)
+    n, npml, order = 256, 20, 4
+    shape = (n, n, n)
+    med = media.layered(shape)
+    s = Solver(n, n, n, 5.0, 4e-4, order=order, precision=precision, npml=npml)
+    s.set_medium(med)
+    o = oracle.OracleRun(n, n, n, 5.0, 4e-4, order=order, mode=mode, npml=npml, medium=med)
+    st = _random_state(shape, 11)
+    names = oracle.FIELDS
+    for f, v in zip(names, st):
+        s.write_field(f, v)
+        o.set_field(f, v)
+    del st
+    s.run(4)
+    for _ in range(4):
+        o.step_v()
+        o.step_t()
+    worst = 0.0
+    for f in names:
+        e = rel_l2(s.field(f, f64=True), o.field(f))
+        worst = max(worst, e)
+        assert e <= tol, (f, e)
+    print(f"  worst field rel L2 = {worst:.3e}")
+    o.close()
+    s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp64"])
+def test_source_and_receiver_indexing_bit_exact_full_size(precision):
+    """256^3 with 65536 surface receivers: zero coefficients leave the fields as written,
+    so every trace sample must equal the written pattern at its cell, bit for bit, and the
+    source cell must accumulate exactly the injected samples."""
+    print(f"indexing {precision}", flush=True)
+    cfg = configs.layered256(order=4, nt=4)
+    n = cfg["nx"]
+    s = Solver(n, n, n, 5.0, 4e-4, order=4, precision=precision, npml=20)
+    zeros = [np.zeros(n ** 3, np.float32) for _ in range(23)]
+    s.set_coefficients(zeros)
+    k_, j_, i_ = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    pat = ((i_ * 7 + j_ * 3 + k_ * 5) % 1024).astype(np.float64)  # exact in binary16
+    for f in ("vx", "vz", "p", "txx", "tyy", "tzz"):
+        s.write_field(f, pat)
+    src = configs.cells(cfg, [(37, 91, 200)])[0]
+    w = np.array([1.0, 2.0, -3.0, 0.5], np.float32)
+    s.set_source(src, w, ("txx", "p"))
+    rec = configs.cells(cfg, cfg["rec"])
+    s.set_receivers(rec, ("vx", "vz", "p"), 4)
+    s.run(4)
+    tr = s.traces()
+    flat = pat.reshape(-1)
+    for c in range(3):
+        for it in range(4):
+            assert np.array_equal(tr[c, it], flat[rec].astype(np.float32))
+    p = s.field("p").reshape(-1)
+    txx = s.field("txx").reshape(-1)
+    assert p[src] == flat[src] + w.sum() and txx[src] == flat[src] + w.sum()
+    mask = np.ones(n ** 3, bool)
+    mask[src] = False
+    assert np.array_equal(p[mask], flat[mask].astype(np.float32))
+    assert np.array_equal(s.field("tyy").reshape(-1), flat.astype(np.float32))
+    s.close()
+
+
+def test_snapshot_reset_determinism_and_errors():
+    cfg = configs.fwd64(order=4, nt=120)
+    s = configs.make_solver(cfg, nt=120)
+    s.run(120)
+    t1 = s.traces()
+    snap = s.snapshot_xz("txx", 32)
+    assert np.array_equal(snap, s.field("txx")[:, 32, :])
+    s.reset()
+    assert not np.any(s.field("vx"))
+    s.run(120)
+    assert np.array_equal(t1, s.traces())
+    with pytest.raises(BiotError) as e:
+        s.set_receivers([64 ** 3], ("p",), 10)
+    assert e.value.status == 1
+    with pytest.raises(BiotError) as e:
+        s.snapshot_xz("p", 64)
+    assert e.value.status == 1
+    s.close()
+    t = Solver(32, 32, 32, 5.0, 4e-4, order=2)
+    with pytest.raises(BiotError) as e:
+        t.run(1)
+    assert e.value.status == 3
+    t.close()

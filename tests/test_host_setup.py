@@ -1,0 +1,109 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports every symbol its
+header declares; its set-up stages (coefficients, PML profiles, Ricker) equal the oracle's
+bit for bit; argument errors come back as status codes."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from biot_amd import _lib, media
+from biot_amd._lib import BIOT_OK, Grid
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(inc, fn)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(biot_\w+)\s*\(", txt, re.M))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    decl = declared_functions()
+    assert len(decl) >= 25
+    missing = [n for n in decl if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(_lib.exported_symbols()) <= set(decl)
+    assert L.biot_abi_version() == 1
+
+
+def _oracle_coefs(medium, nx, ny, nz, dt):
+    shape = (nz, ny, nx)
+    mod = oracle.biot_moduli(medium, shape)
+    av = oracle.stagger_average(mod, nx, ny, nz)
+    return oracle.coefficients(mod, av, nx, ny, nz, dt)
+
+
+@pytest.mark.parametrize("kind", ["homogeneous", "layered", "perturbed", "borehole"])
+def test_coefficients_bit_exact_vs_oracle(kind):
+    nx, ny, nz, dt = 20, 18, 24, 4e-4
+    shape = (nz, ny, nx)
+    if kind == "homogeneous":
+        m = media.homogeneous(shape)
+    elif kind == "layered":
+        m = media.layered(shape)
+    elif kind == "perturbed":
+        m = media.perturbed(shape, corr_cells=3.0, seed=7)
+    else:  # fluid column as in the reference's borehole model (mu = Kb = 0)
+        m = media.layered(shape)
+        jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        hole = ((ii - nx // 2) ** 2 + (jj - ny // 2) ** 2) <= 9
+        for k in ("mu", "Kb"):
+            m[k][:, hole] = 0.0
+    ours = _lib.build_coefficients(nx, ny, nz, dt, m)
+    ref = _oracle_coefs(m, nx, ny, nz, dt)
+    for a in range(23):
+        assert np.array_equal(ours[a].view(np.uint32), ref[a].view(np.uint32)), _lib.COEF_NAMES[a]
+
+
+@pytest.mark.parametrize("n,npml,dx,dt", [(64, 10, 5.0, 4e-4), (256, 20, 5.0, 4e-4),
+                                          (240, 30, 0.01, 1e-6), (33, 7, 2.5, 1e-4)])
+def test_pml_profile_bit_exact_vs_oracle(n, npml, dx, dt):
+    ours = _lib.pml_profile(n, dx, dt, npml)
+    ref = oracle.pml_profile(n, dx, dt, npml)
+    for a, b in zip(ours, ref):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("f0,dt,nt", [(25.0, 4e-4, 500), (20.0, 4e-4, 2000), (1e4, 1e-6, 3500)])
+def test_ricker_bit_exact_vs_oracle(f0, dt, nt):
+    ours = _lib.ricker(f0, dt, nt)
+    ref = oracle.ricker_series(nt, dt, f0)
+    assert np.array_equal(ours.view(np.uint32), ref.view(np.uint32))
+
+
+def test_argument_errors_are_status_codes():
+    L = _lib.lib()
+    h = C.c_void_p()
+    bad = Grid(nx=64, ny=64, nz=64, order=3, precision=0, npml=10, dx=5, dt=4e-4, pml_v0=4000,
+               pml_R=1e-6, Cv=1e8, Cp=1, device=0)
+    assert L.biot_create(C.byref(bad), C.byref(h)) == 1  # BIOT_ERR_ARG
+    bad.order, bad.nx = 4, 4
+    assert L.biot_create(C.byref(bad), C.byref(h)) == 1
+    bad.nx, bad.npml = 64, 40
+    assert L.biot_create(C.byref(bad), C.byref(h)) == 1
+    bad.npml, bad.precision = 10, 7
+    assert L.biot_create(C.byref(bad), C.byref(h)) == 1
+    assert L.biot_ricker(C.c_float(-1.0), C.c_float(1e-3), 4, None) == 1
+    assert L.biot_status_string(4) == b"non-finite value detected"
+
+
+def test_create_without_device_fails_loudly():
+    """Without a GPU the library reports a HIP error instead of falling back to the CPU."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    g = Grid(nx=16, ny=16, nz=16, order=2, precision=0, npml=2, dx=5, dt=4e-4, pml_v0=4000,
+             pml_R=1e-6, Cv=1e8, Cp=1, device=0)
+    rc = L.biot_create(C.byref(g), C.byref(h))
+    if rc == BIOT_OK:
+        L.biot_destroy(h)
+        pytest.skip("a GPU is present")
+    assert rc == 2  # BIOT_ERR_HIP
