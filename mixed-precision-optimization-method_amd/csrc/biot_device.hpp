@@ -72,6 +72,7 @@ struct RecordArgs {
   int it, nt_max;
 };
 hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s);
+hipError_t launch_rescale(int prec, void *p, int64_t n, double f, hipStream_t s);
 hipError_t launch_record(int prec, const RecordArgs &a, hipStream_t s);
 
 }  // namespace biot

@@ -76,8 +76,13 @@ int biot_handle::upload_profiles()
       const float e_s = prof[ax][2][i] * Cpml, e2_s = prof[ax][3][i] * Cpml;
       ei[i] = (double)e_s * 100.0;
       eh[i] = (double)e2_s * 100.0;
-      ai[i] = (double)(float)(-g.dt * d_s) * 100.0 * 0.5;
-      ah[i] = (double)(float)(-g.dt * d2_s) * 100.0 * 0.5;
+      if (prec == BIOT_FP64) {  // no float rounding of -DT*d on the double path
+        ai[i] = -(double)g.dt * (double)d_s * 100.0 * 0.5;
+        ah[i] = -(double)g.dt * (double)d2_s * 100.0 * 0.5;
+      } else {                  // float product first, as the reference's expression
+        ai[i] = (double)(float)(-g.dt * d_s) * 100.0 * 0.5;
+        ah[i] = (double)(float)(-g.dt * d2_s) * 100.0 * 0.5;
+      }
     }
     const std::vector<double> *src[4] = {&ei, &ai, &eh, &ah};
     for (int q = 0; q < 4; ++q) {
@@ -148,9 +153,12 @@ int biot_handle::upload_coefficients()
 {
   std::vector<double> K[NKC];
   for (auto &v : K) v.assign((size_t)n, 0.0);
-  KernCtx cx{coef_ref, 1.0 / (double)g.dx, K};
+  // 1/dx as the float the reference's kernels hold (`float H`, kernel.cu:158)
+  KernCtx cx{coef_ref, (double)(1.0f / g.dx), K};
   parallel_for(n, kern_body, &cx);
 
+  float old_scale[NFIELD];
+  std::memcpy(old_scale, fscale, sizeof old_scale);
   for (int f = 0; f < NFIELD; ++f) fscale[f] = 1.0f;
   for (int k = 0; k < NKC; ++k) ks[k] = 1.0f;
   if (prec == BIOT_FP16) {
@@ -200,6 +208,30 @@ int biot_handle::upload_coefficients()
         scale_arr(k, 1.0 / (double)ks[k]);
       }
     }
+  }
+
+  // Stored wavefields and C-PML carries are expressed in the storage scales: when a scale
+  // changes (e.g. a new source amplitude), re-express what the handle already holds.
+  // Carries 0..3 of each axis are differences of stresses/pressure, 4..7 of solid
+  // velocities except carry 5 (relative fluid velocity).
+  auto rescale = [&](void *p, int64_t cnt, double f) -> bool {
+    return f == 1.0 || launch_rescale(prec, p, cnt, f, stream) == hipSuccess;
+  };
+  bool ok = true;
+  for (int f = 0; f < NFIELD && ok; ++f)
+    ok = rescale(field_ptr(f), n, (double)old_scale[f] / (double)fscale[f]);
+  const double rt = (double)old_scale[6] / fscale[6], rv = (double)old_scale[0] / fscale[0];
+  const double rq = (double)old_scale[3] / fscale[3];
+  for (int ax = 0; ax < 3 && ok; ++ax) {
+    if (slab[ax] == 0) continue;
+    for (int c = 0; c < NCARRY && ok; ++c) {
+      const double f = c < 4 ? rt : (c == 5 ? rq : rv);
+      ok = rescale((char *)d_carry[ax] + (size_t)c * slab[ax] * esz, slab[ax], f);
+    }
+  }
+  if (!ok || hipStreamSynchronize(stream) != hipSuccess) {
+    err = "rescaling the stored wavefields failed";
+    return BIOT_ERR_HIP;
   }
 
   std::vector<unsigned char> buf((size_t)n * esz);

@@ -10,6 +10,7 @@
 // variable described in biot_device.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "biot_device.hpp"
@@ -390,6 +391,19 @@ __global__ void k_record(const RecordArgs a)
   a.traces[((int64_t)ci * a.nt_max + a.it) * a.nrec + r] = v;
 }
 
+// In-place multiply of a stored array (re-expresses values when a storage scale changes).
+template <typename S>
+__global__ void k_rescale(S *p, int64_t n, double f)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if constexpr (std::is_same<S, double>::value)
+      p[i] = p[i] * f;
+    else
+      p[i] = (S)((float)p[i] * (float)f);
+  }
+}
+
 template <typename S, int ORDER, bool VEL>
 hipError_t launch_step(const void *args, LaunchShape sh, hipStream_t s)
 {
@@ -439,6 +453,17 @@ hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s)
   if (prec == 0) hipLaunchKernelGGL(k_source<float>, dim3(1), dim3(64), 0, s, a);
   else if (prec == 1) hipLaunchKernelGGL(k_source<_Float16>, dim3(1), dim3(64), 0, s, a);
   else if (prec == 2) hipLaunchKernelGGL(k_source<double>, dim3(1), dim3(64), 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_rescale(int prec, void *p, int64_t n, double f, hipStream_t s)
+{
+  if (n <= 0 || !p) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+  if (prec == 0) hipLaunchKernelGGL(k_rescale<float>, dim3(blocks), dim3(256), 0, s, (float *)p, n, f);
+  else if (prec == 1) hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, n, f);
+  else if (prec == 2) hipLaunchKernelGGL(k_rescale<double>, dim3(blocks), dim3(256), 0, s, (double *)p, n, f);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }

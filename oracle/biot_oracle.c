@@ -405,11 +405,17 @@ void or_state_get_field_f64(const or_state *st, int field, double *out)
 
 void or_state_set_field_f64(or_state *st, int field, const double *in)
 {
+  /* A relative fluid velocity also sets its previous-step copy: between steps the
+   * reference's vwx2..vwz2 always equal vwx..vwz (kernel.cu:226). */
   const int64_t n = (int64_t)st->nx * st->ny * st->nz;
-  for (int64_t c = 0; c < n; c++) {
-    if (st->mode == OR_MODE_F32) set_f32(st->f[field], c, in[c]);
-    else if (st->mode == OR_MODE_F16) set_f16(st->f[field], c, in[c]);
-    else set_f64(st->f[field], c, in[c]);
+  void *dst[2] = {st->f[field], (field >= 3 && field <= 5) ? st->qprev[field - 3] : NULL};
+  for (int d = 0; d < 2; d++) {
+    if (!dst[d]) continue;
+    for (int64_t c = 0; c < n; c++) {
+      if (st->mode == OR_MODE_F32) set_f32(dst[d], c, in[c]);
+      else if (st->mode == OR_MODE_F16) set_f16(dst[d], c, in[c]);
+      else set_f64(dst[d], c, in[c]);
+    }
   }
 }
 
