@@ -73,6 +73,9 @@ def run_precision(cfg, precision, steps, warmup):
     t0 = time.time()
     s = configs.make_solver(cfg, precision=precision, nt=nt)
     log(f"[bench] {precision}: set-up {time.time() - t0:.1f} s, device {s.device_bytes() / 2**30:.2f} GiB")
+    t0 = time.time()
+    shape = s.autotune(5)
+    log(f"[bench] {precision}: tile {shape} ({time.time() - t0:.1f} s autotune)")
     s.run(warmup)  # biot_run returns after the device has finished
     t0 = time.perf_counter()
     s.run(steps)
@@ -87,7 +90,7 @@ def run_precision(cfg, precision, steps, warmup):
     widths = slab_widths(s)
     s.close()
     return dict(wall_s=wall, event_ms=ev_ms, kernel_ms_v=pv, kernel_ms_t=pt, widths=widths,
-                finite=finite)
+                finite=finite, shape=list(shape))
 
 
 def cpu_baseline(cfg, nsteps):
@@ -116,9 +119,24 @@ def cpu_baseline(cfg, nsteps):
                        f"(-O3, OpenMP), {dt:.1f} s")
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB (torch's copy kernel), repeated 5 times:
+    the HBM-traffic counters of this dispatch calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    n = mib * 2 ** 20 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
@@ -126,6 +144,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     args = ap.parse_args()
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solver holds the whole grid on a single MI355X "
               f"and has no multi-GPU path; got --gpus {args.gpus}", file=sys.stderr)
@@ -160,7 +181,7 @@ def main():
                                f"3-layer Biot medium, 20 Hz Ricker, {len(cfg['rec'])} receivers x "
                                f"(vx, vz, p), FP32 storage",
                    "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "order": 4, "npml": 20,
-                   "dx_m": cfg["dx"], "dt_s": cfg["dt"]},
+                   "dx_m": cfg["dx"], "dt_s": cfg["dt"], "tile_bx_by_zchunk": r32["shape"]},
         "roofline": {"bound": "hbm", "kernel": "k_velocity<float,4>",
                      "achieved": round(ach_v, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_v / HBM_PEAK_GBS, 4), "traffic": None,

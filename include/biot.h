@@ -146,6 +146,17 @@ int biot_last_run_ms(biot_handle *h, double *ms);
  * stencil kernels on the handle's stream; returns their mean durations in ms. */
 int biot_profile_steps(biot_handle *h, int32_t nsteps, double *ms_velocity, double *ms_stress);
 
+/* ---- launch shapes ---- */
+/* Tile shape of the two stencil kernels: bx*by threads per block (bx*by <= 256, bx a
+ * multiple of 64), each thread marching zchunk planes. */
+int biot_set_launch_shape(biot_handle *h, int32_t bx, int32_t by, int32_t zchunk);
+int biot_get_launch_shape(biot_handle *h, int32_t *bx, int32_t *by, int32_t *zchunk);
+/* Times every candidate tile shape for trial_steps steps on this grid and keeps the
+ * fastest; results are cached per (precision, order, grid, GPU arch) in the file named by
+ * $BIOT_TUNE_CACHE (default $HOME/.cache/biot_tune.txt), and a cached entry is reused
+ * without timing.  Leaves the handle reset (wavefields zero, step 0). */
+int biot_autotune(biot_handle *h, int32_t trial_steps);
+
 /* ---- outputs ---- */
 int biot_read_traces(biot_handle *h, float *out, int64_t count);
 int biot_read_field(biot_handle *h, int32_t field, float *out);

@@ -94,6 +94,9 @@ def lib():
         "biot_synchronize": ([H], C.c_int),
         "biot_last_run_ms": ([H, _DP], C.c_int),
         "biot_profile_steps": ([H, i32, _DP, _DP], C.c_int),
+        "biot_set_launch_shape": ([H, i32, i32, i32], C.c_int),
+        "biot_get_launch_shape": ([H, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
+        "biot_autotune": ([H, i32], C.c_int),
         "biot_read_traces": ([H, FP, i64], C.c_int),
         "biot_read_field": ([H, i32, FP], C.c_int),
         "biot_read_field_f64": ([H, i32, _DP], C.c_int),
@@ -118,6 +121,7 @@ def exported_symbols():
             "biot_pml_profile_1d", "biot_set_source", "biot_ricker",
             "biot_set_receivers", "biot_reset", "biot_run", "biot_step_count",
             "biot_synchronize", "biot_last_run_ms", "biot_profile_steps",
+            "biot_set_launch_shape", "biot_get_launch_shape", "biot_autotune",
             "biot_read_traces", "biot_read_field",
             "biot_read_field_f64", "biot_write_field_f64", "biot_snapshot_xz",
             "biot_get_field_scales", "biot_device_bytes"]

@@ -125,6 +125,21 @@ class Solver:
         self._check(lib().biot_last_run_ms(self._h, C.byref(ms)), "biot_last_run_ms")
         return ms.value
 
+    def set_launch_shape(self, bx, by, zchunk):
+        self._check(lib().biot_set_launch_shape(self._h, bx, by, zchunk), "biot_set_launch_shape")
+
+    def launch_shape(self):
+        v = [C.c_int32() for _ in range(3)]
+        self._check(lib().biot_get_launch_shape(self._h, *[C.byref(x) for x in v]),
+                    "biot_get_launch_shape")
+        return tuple(x.value for x in v)
+
+    def autotune(self, trial_steps=5):
+        """Picks the fastest tile shape for this grid (cached per GPU arch); resets the
+        wavefields."""
+        self._check(lib().biot_autotune(self._h, int(trial_steps)), "biot_autotune")
+        return self.launch_shape()
+
     def profile_steps(self, nsteps):
         """Advances nsteps with HIP events around every stencil launch; returns the mean
         (velocity, stress) kernel durations in ms."""

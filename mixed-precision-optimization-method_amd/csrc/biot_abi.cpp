@@ -7,8 +7,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <sys/stat.h>
 #include <vector>
 
 #include "../../include/biot.h"
@@ -211,6 +213,14 @@ extern "C" int biot_create(const biot_grid *g, biot_handle **out)
   h->slab[0] = (int64_t)(h->L[0] + h->R[0]) * g->ny * g->nz;
   h->slab[1] = (int64_t)g->nx * (h->L[1] + h->R[1]) * g->nz;
   h->slab[2] = (int64_t)g->nx * g->ny * (h->L[2] + h->R[2]);
+  // the kernels address every array with 32-bit byte offsets from its base
+  const int64_t lim = (int64_t)1 << 32;
+  bool fits = h->n * (int64_t)h->esz < lim;
+  for (int ax = 0; ax < 3; ++ax) fits = fits && NCARRY * h->slab[ax] * (int64_t)h->esz < lim;
+  if (!fits) {
+    delete h;
+    return BIOT_ERR_ARG;
+  }
   rc = h->allocate();
   if (rc) {
     std::string e = h->err;
@@ -430,10 +440,6 @@ int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks)
   if (prec == BIOT_FP32) fill_args(this, *(StepArgs<float> *)args);
   else if (prec == BIOT_FP16) fill_args(this, *(StepArgs<_Float16> *)args);
   else fill_args(this, *(StepArgs<double> *)args);
-  // zchunk lives at the same offset for every storage type
-  if (prec == BIOT_FP32) ((StepArgs<float> *)args)->zchunk = shape.zchunk;
-  else if (prec == BIOT_FP16) ((StepArgs<_Float16> *)args)->zchunk = shape.zchunk;
-  else ((StepArgs<double> *)args)->zchunk = shape.zchunk;
 
   for (int s = 0; s < nsteps; ++s) {
     if (d_wavelet && src_mask && it < (int)wavelet.size()) {
@@ -508,6 +514,138 @@ extern "C" int biot_profile_steps(biot_handle *h, int32_t nsteps, double *ms_v, 
   *ms_v = sv / nsteps;
   *ms_t = stt / nsteps;
   return BIOT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// launch shapes and the autotuner
+// ------------------------------------------------------------------------------------
+static bool shape_ok(int bx, int by, int zc)
+{
+  return bx >= 64 && bx % 64 == 0 && by >= 1 && bx * by <= 256 && zc >= 1 && zc <= 4096;
+}
+
+extern "C" int biot_set_launch_shape(biot_handle *h, int32_t bx, int32_t by, int32_t zchunk)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (!shape_ok(bx, by, zchunk))
+    return fail(h, BIOT_ERR_ARG, "launch shape %dx%d, zchunk %d not allowed", bx, by, zchunk);
+  h->shape = LaunchShape{bx, by, zchunk};
+  return BIOT_OK;
+}
+
+extern "C" int biot_get_launch_shape(biot_handle *h, int32_t *bx, int32_t *by, int32_t *zchunk)
+{
+  if (!h || !bx || !by || !zchunk) return BIOT_ERR_ARG;
+  *bx = h->shape.bx;
+  *by = h->shape.by;
+  *zchunk = h->shape.zchunk;
+  return BIOT_OK;
+}
+
+namespace {
+std::string tune_cache_path()
+{
+  if (const char *p = std::getenv("BIOT_TUNE_CACHE")) return p;
+  if (const char *home = std::getenv("HOME")) return std::string(home) + "/.cache/biot_tune.txt";
+  return "biot_tune.txt";
+}
+std::string tune_key(const biot_handle *h)
+{
+  hipDeviceProp_t prop;
+  std::string arch = "unknown";
+  if (hipGetDeviceProperties(&prop, h->g.device) == hipSuccess) arch = prop.gcnArchName;
+  for (auto &ch : arch)
+    if (ch == ' ') ch = '_';
+  char buf[256];
+  snprintf(buf, sizeof buf, "v1 %s p%d o%d %dx%dx%d", arch.c_str(), h->prec, h->g.order, h->g.nx,
+           h->g.ny, h->g.nz);
+  return buf;
+}
+bool tune_lookup(const std::string &key, LaunchShape &out)
+{
+  FILE *f = std::fopen(tune_cache_path().c_str(), "r");
+  if (!f) return false;
+  char line[512];
+  bool found = false;
+  while (std::fgets(line, sizeof line, f)) {
+    std::string s(line);
+    if (s.compare(0, key.size(), key) != 0 || s.size() <= key.size() || s[key.size()] != ' ')
+      continue;
+    int bx, by, zc;
+    if (std::sscanf(s.c_str() + key.size(), " %d %d %d", &bx, &by, &zc) == 3 && shape_ok(bx, by, zc)) {
+      out = LaunchShape{bx, by, zc};
+      found = true;  // the last entry for a key wins
+    }
+  }
+  std::fclose(f);
+  return found;
+}
+void tune_store(const std::string &key, LaunchShape s, double ms)
+{
+  const std::string path = tune_cache_path();
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {
+    std::string dir = path.substr(0, slash);
+    // create the directory without a shell: walk the components
+    for (size_t k = 1; k <= dir.size(); ++k)
+      if (k == dir.size() || dir[k] == '/') (void)mkdir(dir.substr(0, k).c_str(), 0755);
+  }
+  if (FILE *f = std::fopen(path.c_str(), "a")) {
+    std::fprintf(f, "%s %d %d %d %.6f\n", key.c_str(), s.bx, s.by, s.zchunk, ms);
+    std::fclose(f);
+  }
+}
+}  // namespace
+
+extern "C" int biot_autotune(biot_handle *h, int32_t trial_steps)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (trial_steps < 1 || trial_steps > 1000) return fail(h, BIOT_ERR_ARG, "trial_steps must be 1..1000");
+  if (!h->have_coef) return fail(h, BIOT_ERR_STATE, "biot_autotune before set_medium");
+  const std::string key = tune_key(h);
+  LaunchShape best{};
+  if (tune_lookup(key, best)) {
+    h->shape = best;
+    return biot_reset(h);
+  }
+  static const int shapes[][2] = {{64, 4}, {64, 2}, {64, 1}, {128, 2}, {128, 1}, {256, 1}};
+  static const int chunks[] = {1, 2, 4, 8, 16, 32, 64};
+  const int nzi = h->g.nz - h->g.order;
+  double best_ms = 1e30;
+  const LaunchShape keep = h->shape;
+  hipEvent_t e0, e1;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  int rc = BIOT_OK;
+  for (auto &sh : shapes) {
+    for (int zc : chunks) {
+      if (zc > 2 * nzi && zc != chunks[0]) continue;
+      h->shape = LaunchShape{sh[0], sh[1], zc};
+      rc = h->enqueue_steps(1, nullptr);  // warm-up
+      if (!rc && hipEventRecord(e0, h->stream) != hipSuccess) rc = BIOT_ERR_HIP;
+      if (!rc) rc = h->enqueue_steps(trial_steps, nullptr);
+      if (!rc && hipEventRecord(e1, h->stream) != hipSuccess) rc = BIOT_ERR_HIP;
+      if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = BIOT_ERR_HIP;
+      float ms = 0;
+      if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = BIOT_ERR_HIP;
+      if (rc) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = h->shape;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    h->shape = keep;
+    (void)hipStreamSynchronize(h->stream);
+    return rc == BIOT_ERR_HIP ? fail(h, rc, "autotune timing failed") : rc;
+  }
+  h->shape = best;
+  tune_store(key, best, best_ms / trial_steps);
+  return biot_reset(h);
 }
 
 extern "C" int biot_step_count(biot_handle *h, int32_t *it)

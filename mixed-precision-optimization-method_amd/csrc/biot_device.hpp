@@ -41,6 +41,7 @@ struct StepArgs {
   int L[3], R[3];            // slab widths
   int64_t sy, sz;            // strides
   int zchunk;
+  int gx, gy, gz, ntiles;    // tile grid (x-y tiles of the block shape, z chunks)
 };
 
 // Host-side launchers (biot_kernels.hip).  prec: 0 fp32, 1 fp16, 2 fp64.
