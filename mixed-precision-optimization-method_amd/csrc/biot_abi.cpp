@@ -43,62 +43,55 @@ int fail(biot_handle *h, int code, const char *fmt, ...)
                   __FILE__, __LINE__);                                                        \
   } while (0)
 
-size_t elem_size(int prec) { return prec == BIOT_FP16 ? 2 : prec == BIOT_FP64 ? 8 : 4; }
+// binary16 residual unit ulp(h)/256 = 2^(e-33), e the biased exponent (>= 1) of h
+double half_res_unit(_Float16 h)
+{
+  uint16_t u;
+  std::memcpy(&u, &h, 2);
+  const int e = std::max((u >> 10) & 31, 1);
+  return std::ldexp(1.0, e - 33);
+}
 
-// host storage-type conversion
-void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst)
+// host storage-type conversion; lo: FP16C residuals of a compensated field, else null
+void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst, int8_t *lo)
 {
   if (prec == BIOT_FP32) {
     float *d = (float *)dst;
     for (int64_t c = 0; c < n; ++c) d[c] = (float)(src[c] * scale_inv);
-  } else if (prec == BIOT_FP16) {
+  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C) {
     _Float16 *d = (_Float16 *)dst;
-    for (int64_t c = 0; c < n; ++c) d[c] = (_Float16)(float)(src[c] * scale_inv);
+    for (int64_t c = 0; c < n; ++c) {
+      const float x = (float)(src[c] * scale_inv);
+      d[c] = (_Float16)x;
+      if (lo) {
+        const double r = std::nearbyint(((double)x - (double)(float)d[c]) / half_res_unit(d[c]));
+        lo[c] = (int8_t)std::min(127.0, std::max(-127.0, r));
+      }
+    }
   } else {
     double *d = (double *)dst;
     for (int64_t c = 0; c < n; ++c) d[c] = src[c] * scale_inv;
   }
 }
-void from_storage(int prec, const void *src, int64_t n, double scale, double *dst)
+void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double scale, double *dst)
 {
   if (prec == BIOT_FP32) {
     const float *s = (const float *)src;
     for (int64_t c = 0; c < n; ++c) dst[c] = (double)s[c] * scale;
-  } else if (prec == BIOT_FP16) {
+  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C) {
     const _Float16 *s = (const _Float16 *)src;
-    for (int64_t c = 0; c < n; ++c) dst[c] = (double)(float)s[c] * scale;
+    for (int64_t c = 0; c < n; ++c) {
+      double v = (double)(float)s[c];
+      if (lo) v += (double)lo[c] * half_res_unit(s[c]);
+      dst[c] = v * scale;
+    }
   } else {
     const double *s = (const double *)src;
     for (int64_t c = 0; c < n; ++c) dst[c] = s[c] * scale;
   }
 }
 
-template <typename S>
-void fill_args(biot_handle *h, StepArgs<S> &a)
-{
-  std::memset(&a, 0, sizeof a);
-  for (int f = 0; f < NFIELD; ++f) a.f[f] = (S *)h->field_ptr(f);
-  for (int k = 0; k < NKC; ++k) {
-    a.k[k] = (const S *)((char *)h->d_coef + (size_t)k * h->n * h->esz);
-    a.ks[k] = h->ks[k];
-  }
-  for (int ax = 0; ax < 3; ++ax) {
-    a.carry[ax] = (S *)h->d_carry[ax];
-    a.slab[ax] = h->slab[ax];
-    a.L[ax] = h->L[ax];
-    a.R[ax] = h->R[ax];
-    for (int q = 0; q < 4; ++q) a.prof[ax][q] = h->d_prof[ax][q];
-  }
-  a.nx = h->g.nx;
-  a.ny = h->g.ny;
-  a.nz = h->g.nz;
-  a.sy = h->g.nx;
-  a.sz = (int64_t)h->g.nx * h->g.ny;
-}
-
 }  // namespace
-
-void *biot_handle::field_ptr(int f) const { return (char *)d_fields + (size_t)f * n * esz; }
 
 // ------------------------------------------------------------------------------------
 // lifetime
@@ -143,8 +136,8 @@ static int validate_grid(const biot_grid *g, std::string &why)
     why = "grid too large";
     return BIOT_ERR_ARG;
   }
-  if (g->precision < 0 || g->precision > 2) {
-    why = "precision must be BIOT_FP32, BIOT_FP16 or BIOT_FP64";
+  if (g->precision < 0 || g->precision > 3) {
+    why = "precision must be BIOT_FP32, BIOT_FP16, BIOT_FP64 or BIOT_FP16C";
     return BIOT_ERR_ARG;
   }
   if (!(g->dx > 0) || !(g->dt > 0) || !(g->Cv > 0) || !(g->Cp > 0)) {
@@ -172,7 +165,8 @@ extern "C" int biot_create(const biot_grid *g, biot_handle **out)
   biot_handle *h = new biot_handle();
   h->g = *g;
   h->prec = g->precision;
-  h->esz = elem_size(g->precision);
+  h->esz = field_bytes(g->precision);
+  h->csz = carry_bytes(g->precision);
   h->n = (int64_t)g->nx * g->ny * g->nz;
   for (int k = 0; k < NKC; ++k) h->ks[k] = 1.0f;
   for (int f = 0; f < NFIELD; ++f) h->fscale[f] = 1.0f;
@@ -216,7 +210,7 @@ extern "C" int biot_create(const biot_grid *g, biot_handle **out)
   // the kernels address every array with 32-bit byte offsets from its base
   const int64_t lim = (int64_t)1 << 32;
   bool fits = h->n * (int64_t)h->esz < lim;
-  for (int ax = 0; ax < 3; ++ax) fits = fits && NCARRY * h->slab[ax] * (int64_t)h->esz < lim;
+  for (int ax = 0; ax < 3; ++ax) fits = fits && NCARRY * h->slab[ax] * (int64_t)h->csz < lim;
   if (!fits) {
     delete h;
     return BIOT_ERR_ARG;
@@ -356,7 +350,7 @@ extern "C" int biot_set_source(biot_handle *h, int64_t cell, uint32_t mask, cons
     HIPCHK(h, hipMalloc(&h->d_wavelet, (size_t)nt * sizeof(float)));
     HIPCHK(h, hipMemcpy(h->d_wavelet, w, (size_t)nt * sizeof(float), hipMemcpyHostToDevice));
   }
-  if (h->prec == BIOT_FP16 && h->have_coef) return h->upload_coefficients();  // rescale
+  if (h->half() && h->have_coef) return h->upload_coefficients();  // storage scales follow |w|
   return BIOT_OK;
 }
 
@@ -401,9 +395,10 @@ extern "C" int biot_reset(biot_handle *h)
 {
   if (!h) return BIOT_ERR_ARG;
   HIPCHK(h, hipMemsetAsync(h->d_fields, 0, (size_t)NFIELD * h->n * h->esz, h->stream));
+  if (h->d_lo) HIPCHK(h, hipMemsetAsync(h->d_lo, 0, (size_t)NFIELD * h->n, h->stream));
   for (int ax = 0; ax < 3; ++ax)
     if (h->slab[ax] > 0)
-      HIPCHK(h, hipMemsetAsync(h->d_carry[ax], 0, (size_t)NCARRY * h->slab[ax] * h->esz, h->stream));
+      HIPCHK(h, hipMemsetAsync(h->d_carry[ax], 0, (size_t)NCARRY * h->slab[ax] * h->csz, h->stream));
   if (h->d_traces)
     HIPCHK(h, hipMemsetAsync(h->d_traces, 0, (size_t)h->ncomp * h->nt_max * h->nrec * sizeof(float),
                              h->stream));
@@ -421,6 +416,8 @@ int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks)
   for (int f = 0; f < NFIELD; ++f) {
     sa.f[f] = field_ptr(f);
     ra.f[f] = field_ptr(f);
+    sa.lo[f] = lo_ptr(f);
+    ra.lo[f] = lo_ptr(f);
     sa.inv_scale[f] = 1.0f / fscale[f];
   }
   sa.cell = src_cell;
@@ -436,10 +433,8 @@ int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks)
   ra.traces = (float *)d_traces;
   ra.nt_max = nt_max;
 
-  alignas(16) unsigned char args[sizeof(StepArgs<double>)];
-  if (prec == BIOT_FP32) fill_args(this, *(StepArgs<float> *)args);
-  else if (prec == BIOT_FP16) fill_args(this, *(StepArgs<_Float16> *)args);
-  else fill_args(this, *(StepArgs<double> *)args);
+  StepArgs args;
+  fill_args(args);
 
   for (int s = 0; s < nsteps; ++s) {
     if (d_wavelet && src_mask && it < (int)wavelet.size()) {
@@ -690,9 +685,14 @@ extern "C" int biot_read_field_f64(biot_handle *h, int32_t field, double *out)
   if (!h || !out) return BIOT_ERR_ARG;
   if (field < 0 || field >= NFIELD) return fail(h, BIOT_ERR_ARG, "field %d out of range", field);
   std::vector<unsigned char> tmp((size_t)h->n * h->esz);
+  std::vector<int8_t> lo;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(tmp.data(), h->field_ptr(field), tmp.size(), hipMemcpyDeviceToHost));
-  from_storage(h->prec, tmp.data(), h->n, h->fscale[field], out);
+  if (h->lo_ptr(field)) {
+    lo.resize((size_t)h->n);
+    HIPCHK(h, hipMemcpy(lo.data(), h->lo_ptr(field), lo.size(), hipMemcpyDeviceToHost));
+  }
+  from_storage(h->prec, tmp.data(), lo.empty() ? nullptr : lo.data(), h->n, h->fscale[field], out);
   return BIOT_OK;
 }
 
@@ -711,9 +711,12 @@ extern "C" int biot_write_field_f64(biot_handle *h, int32_t field, const double 
   if (!h || !in) return BIOT_ERR_ARG;
   if (field < 0 || field >= NFIELD) return fail(h, BIOT_ERR_ARG, "field %d out of range", field);
   std::vector<unsigned char> tmp((size_t)h->n * h->esz);
-  to_storage(h->prec, in, h->n, 1.0 / h->fscale[field], tmp.data());
+  std::vector<int8_t> lo(h->lo_ptr(field) ? (size_t)h->n : 0);
+  to_storage(h->prec, in, h->n, 1.0 / h->fscale[field], tmp.data(), lo.empty() ? nullptr : lo.data());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(h->field_ptr(field), tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+  if (!lo.empty())
+    HIPCHK(h, hipMemcpy(h->lo_ptr(field), lo.data(), lo.size(), hipMemcpyHostToDevice));
   return BIOT_OK;
 }
 
@@ -730,8 +733,15 @@ extern "C" int biot_snapshot_xz(biot_handle *h, int32_t field, int32_t j, float 
                         (char *)h->field_ptr(field) + (size_t)j * nx * h->esz,
                         (size_t)nx * h->g.ny * h->esz, (size_t)nx * h->esz, (size_t)nz,
                         hipMemcpyDeviceToHost));
+  std::vector<int8_t> lo;
+  if (h->lo_ptr(field)) {
+    lo.resize((size_t)nx * nz);
+    HIPCHK(h, hipMemcpy2D(lo.data(), (size_t)nx, h->lo_ptr(field) + (size_t)j * nx,
+                          (size_t)nx * h->g.ny, (size_t)nx, (size_t)nz, hipMemcpyDeviceToHost));
+  }
   std::vector<double> tmp((size_t)nx * nz);
-  from_storage(h->prec, row.data(), (int64_t)nx * nz, h->fscale[field], tmp.data());
+  from_storage(h->prec, row.data(), lo.empty() ? nullptr : lo.data(), (int64_t)nx * nz,
+               h->fscale[field], tmp.data());
   for (size_t c = 0; c < tmp.size(); ++c) out[c] = (float)tmp[c];
   return BIOT_OK;
 }

@@ -14,6 +14,15 @@
 //                 psi_n = c + a D_n,   D <- D + psi_n,   c <- e (psi_n + a D_n)
 //             which is algebraically the recursion of kernel.cu:188-196.
 //   profiles: per axis e and a = -dt d / 2 at integer and half nodes (1-D, arithmetic type).
+//   residuals (FP16C only): one int8 array per compensated field (v, tau, p).
+//
+// Storage modes (enum biot_precision):
+//   FP32  fields, coefficients, carries float; arithmetic float.
+//   FP16  fields and coefficients binary16 (scaled), carries float; arithmetic float.
+//   FP16C as FP16, plus an int8 residual r per value of v, tau and p: the value is
+//         h + r * ulp(h) / 256 (h the binary16 part), about a 19-bit significand, so the
+//         rounding of the leapfrog-integrated fields does not random-walk.
+//   FP64  everything double.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -22,6 +31,9 @@
 namespace biot {
 
 enum { NFIELD = 13, NKC = 22, NCARRY = 8 };
+enum { PREC_FP32 = 0, PREC_FP16 = 1, PREC_FP64 = 2, PREC_FP16C = 3 };
+// fields carrying an FP16C residual: vx vy vz txx tyy tzz txy txz tyz p
+constexpr uint32_t COMP_FIELDS = 0x1FC7u;
 // kernel coefficient order
 enum {
   K_U1X = 0, K_U1Y, K_U1Z, K_U2X, K_U2Y, K_U2Z, K_W1X, K_W1Y, K_W1Z, K_W2X, K_W2Y, K_W2Z,
@@ -29,12 +41,13 @@ enum {
   K_P1, K_P2, K_S1, K_S2, K_S3, K_SXY, K_SXZ, K_SYZ
 };
 
-template <typename S>
+// Kernel arguments; pointers are type-erased, their element types follow the storage mode.
 struct StepArgs {
-  S *f[NFIELD];
-  const S *k[NKC];
+  void *f[NFIELD];           // field storage
+  int8_t *lo[NFIELD];        // FP16C residuals (null for other modes / fields)
+  const void *k[NKC];        // coefficient arrays (field storage type)
   float ks[NKC];             // per-array scale (binary16 storage), 1 otherwise
-  S *carry[3];               // per axis: NCARRY arrays of slab[ax] elements
+  void *carry[3];            // per axis: NCARRY arrays of slab[ax] elements (carry type)
   int64_t slab[3];           // elements per carry array per axis
   const void *prof[3][4];    // per axis: e_int, a_int, e_half, a_half (arithmetic type)
   int nx, ny, nz;
@@ -44,18 +57,21 @@ struct StepArgs {
   int gx, gy, gz, ntiles;    // tile grid (x-y tiles of the block shape, z chunks)
 };
 
-// Host-side launchers (biot_kernels.hip).  prec: 0 fp32, 1 fp16, 2 fp64.
-// Storage pointers are passed type-erased as the matching StepArgs<S>.
+inline size_t field_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP32 ? 4 : 2; }
+inline size_t carry_bytes(int prec) { return prec == PREC_FP64 ? 8 : 4; }
+
+// Host-side launchers (biot_kernels.hip).
 struct LaunchShape {
   int bx, by, zchunk;
 };
 
-hipError_t launch_velocity(int prec, int order, const void *args, LaunchShape shape,
+hipError_t launch_velocity(int prec, int order, const StepArgs &args, LaunchShape shape,
                            hipStream_t s);
-hipError_t launch_stress(int prec, int order, const void *args, LaunchShape shape,
+hipError_t launch_stress(int prec, int order, const StepArgs &args, LaunchShape shape,
                          hipStream_t s);
 struct SourceArgs {
   void *f[NFIELD];
+  int8_t *lo[NFIELD];
   int64_t cell;
   uint32_t mask;
   const float *wavelet;     // device, one sample per step
@@ -64,6 +80,7 @@ struct SourceArgs {
 };
 struct RecordArgs {
   const void *f[NFIELD];
+  const int8_t *lo[NFIELD];
   const int64_t *rec;       // device receiver cells
   int64_t nrec;
   int comp[NFIELD];
@@ -73,7 +90,9 @@ struct RecordArgs {
   int it, nt_max;
 };
 hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s);
-hipError_t launch_rescale(int prec, void *p, int64_t n, double f, hipStream_t s);
+// multiply a stored array by f (lo: its FP16C residuals or null); carries use prec of the
+// carry type, i.e. pass PREC_FP32 for FP16/FP16C carries
+hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
 hipError_t launch_record(int prec, const RecordArgs &a, hipStream_t s);
 
 }  // namespace biot

@@ -12,6 +12,7 @@ biot_handle::~biot_handle()
 {
   if (stream) (void)hipStreamSynchronize(stream);
   if (d_fields) (void)hipFree(d_fields);
+  if (d_lo) (void)hipFree(d_lo);
   if (d_coef) (void)hipFree(d_coef);
   for (auto *p : d_carry)
     if (p) (void)hipFree(p);
@@ -24,6 +25,39 @@ biot_handle::~biot_handle()
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+void *biot_handle::field_ptr(int f) const { return (char *)d_fields + (size_t)f * n * esz; }
+
+int8_t *biot_handle::lo_ptr(int f) const
+{
+  if (!d_lo || !((COMP_FIELDS >> f) & 1u)) return nullptr;
+  return d_lo + (size_t)f * n;
+}
+
+void biot_handle::fill_args(StepArgs &a) const
+{
+  std::memset(&a, 0, sizeof a);
+  for (int f = 0; f < NFIELD; ++f) {
+    a.f[f] = field_ptr(f);
+    a.lo[f] = lo_ptr(f);
+  }
+  for (int k = 0; k < NKC; ++k) {
+    a.k[k] = (const char *)d_coef + (size_t)k * n * esz;
+    a.ks[k] = ks[k];
+  }
+  for (int ax = 0; ax < 3; ++ax) {
+    a.carry[ax] = d_carry[ax];
+    a.slab[ax] = slab[ax];
+    a.L[ax] = L[ax];
+    a.R[ax] = R[ax];
+    for (int q = 0; q < 4; ++q) a.prof[ax][q] = d_prof[ax][q];
+  }
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.nz = g.nz;
+  a.sy = g.nx;
+  a.sz = (int64_t)g.nx * g.ny;
 }
 
 LaunchShape default_shape(int prec, int order)
@@ -45,7 +79,8 @@ int biot_handle::allocate()
     return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
   };
   bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, (size_t)NKC * n * esz);
-  for (int ax = 0; ax < 3 && ok; ++ax) ok = alloc(&d_carry[ax], (size_t)NCARRY * slab[ax] * esz);
+  if (ok && prec == BIOT_FP16C) ok = alloc((void **)&d_lo, (size_t)NFIELD * n);
+  for (int ax = 0; ax < 3 && ok; ++ax) ok = alloc(&d_carry[ax], (size_t)NCARRY * slab[ax] * csz);
   const int dims[3] = {g.nx, g.ny, g.nz};
   const size_t asz = prec == BIOT_FP64 ? 8 : 4;
   for (int ax = 0; ax < 3 && ok; ++ax)
@@ -161,7 +196,7 @@ int biot_handle::upload_coefficients()
   std::memcpy(old_scale, fscale, sizeof old_scale);
   for (int f = 0; f < NFIELD; ++f) fscale[f] = 1.0f;
   for (int k = 0; k < NKC; ++k) ks[k] = 1.0f;
-  if (prec == BIOT_FP16) {
+  if (half()) {
     double A = 0;
     for (float w : wavelet) A = std::max(A, (double)std::fabs(w));
     if (!(A > 0)) A = 1.0;
@@ -214,19 +249,20 @@ int biot_handle::upload_coefficients()
   // changes (e.g. a new source amplitude), re-express what the handle already holds.
   // Carries 0..3 of each axis are differences of stresses/pressure, 4..7 of solid
   // velocities except carry 5 (relative fluid velocity).
-  auto rescale = [&](void *p, int64_t cnt, double f) -> bool {
-    return f == 1.0 || launch_rescale(prec, p, cnt, f, stream) == hipSuccess;
+  auto rescale = [&](int p_, void *p, int8_t *lo, int64_t cnt, double f) -> bool {
+    return f == 1.0 || launch_rescale(p_, p, lo, cnt, f, stream) == hipSuccess;
   };
   bool ok = true;
   for (int f = 0; f < NFIELD && ok; ++f)
-    ok = rescale(field_ptr(f), n, (double)old_scale[f] / (double)fscale[f]);
+    ok = rescale(prec, field_ptr(f), lo_ptr(f), n, (double)old_scale[f] / (double)fscale[f]);
   const double rt = (double)old_scale[6] / fscale[6], rv = (double)old_scale[0] / fscale[0];
   const double rq = (double)old_scale[3] / fscale[3];
   for (int ax = 0; ax < 3 && ok; ++ax) {
     if (slab[ax] == 0) continue;
     for (int c = 0; c < NCARRY && ok; ++c) {
       const double f = c < 4 ? rt : (c == 5 ? rq : rv);
-      ok = rescale((char *)d_carry[ax] + (size_t)c * slab[ax] * esz, slab[ax], f);
+      ok = rescale(prec == BIOT_FP64 ? BIOT_FP64 : BIOT_FP32,
+                   (char *)d_carry[ax] + (size_t)c * slab[ax] * csz, nullptr, slab[ax], f);
     }
   }
   if (!ok || hipStreamSynchronize(stream) != hipSuccess) {
@@ -240,7 +276,7 @@ int biot_handle::upload_coefficients()
     if (prec == BIOT_FP32) {
       float *d = (float *)buf.data();
       for (int64_t c = 0; c < n; ++c) d[c] = (float)src[c];
-    } else if (prec == BIOT_FP16) {
+    } else if (half()) {
       _Float16 *d = (_Float16 *)buf.data();
       for (int64_t c = 0; c < n; ++c) d[c] = (_Float16)(float)src[c];
     } else {

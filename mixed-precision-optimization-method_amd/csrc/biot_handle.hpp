@@ -12,7 +12,8 @@
 struct biot_handle {
   biot_grid g{};
   int prec = 0;
-  size_t esz = 4;
+  size_t esz = 4;    // field / coefficient element bytes
+  size_t csz = 4;    // C-PML carry element bytes
   int64_t n = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -20,6 +21,7 @@ struct biot_handle {
 
   // device memory
   void *d_fields = nullptr;          // 13 * n
+  int8_t *d_lo = nullptr;            // FP16C residuals, 13 * n (compensated fields used)
   void *d_coef = nullptr;            // 22 * n
   void *d_carry[3] = {nullptr, nullptr, nullptr};
   void *d_prof[3][4] = {};           // e_int, a_int, e_half, a_half per axis
@@ -54,6 +56,9 @@ struct biot_handle {
 
   ~biot_handle();
   void *field_ptr(int f) const;
+  int8_t *lo_ptr(int f) const;  // residual array of field f, or null
+  bool half() const { return prec == BIOT_FP16 || prec == BIOT_FP16C; }
+  void fill_args(biot::StepArgs &a) const;
   int allocate();
   int upload_profiles();
   int upload_coefficients();
