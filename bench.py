@@ -34,24 +34,47 @@ def slab_counts(n, L, R, h):
     return lo + hi
 
 
-def algorithmic_bytes(nx, ny, nz, order, npml_cells, w):
+# element bytes per storage mode: field, coefficient, C-PML carry, residual (FP16C only,
+# on the 10 compensated fields v, tau, p)
+WIDTHS = {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 4, 0), "fp16c": (2, 2, 4, 1), "fp64": (8, 8, 8, 0)}
+
+
+def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):
     """Compulsory HBM bytes per launch of the velocity and stress kernels and per step.
 
     Velocity kernel per updated cell: reads 7 stress/pressure + 6 velocity fields, writes 6,
     reads 14 coefficient arrays; in an absorbing slab of one axis it reads and writes 4
     carries.  Stress kernel: reads 6 + 7, writes 7, reads 8 coefficients, 4 carries per
     slab.  The fused-step figure is 26 field + 22 coefficient accesses per cell plus
-    16 carry accesses per slab membership (one read and one write of 8 carries)."""
+    16 carry accesses per slab membership (one read and one write of 8 carries); FP16C adds
+    one residual byte per access of v, tau, p (V: 10 reads + 3 writes, T: 10 + 7, step 20)."""
+    wf, wk, wc, wr = WIDTHS[mode]
     h = order // 2
     ni, nj, nk = nx - 2 * h, ny - 2 * h, nz - 2 * h
     upd = ni * nj * nk
     Lx, Rx, Ly, Ry, Lz, Rz = npml_cells
     memb = (slab_counts(nx, Lx, Rx, h) * nj * nk + slab_counts(ny, Ly, Ry, h) * ni * nk +
             slab_counts(nz, Lz, Rz, h) * ni * nj)
-    v = upd * (19 + 14) * w + memb * 8 * w
-    t = upd * (20 + 8) * w + memb * 8 * w
-    step = upd * (26 + 22) * w + memb * 16 * w
+    v = upd * (19 * wf + 14 * wk + 13 * wr) + memb * 8 * wc
+    t = upd * (20 * wf + 8 * wk + 17 * wr) + memb * 8 * wc
+    step = upd * (26 * wf + 22 * wk + 20 * wr) + memb * 16 * wc
     return v, t, step, upd, memb
+
+
+PROFILE = os.path.join(ROOT, "profiles", "stencil_kernels_layered256.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (counters scaled by a 1 GiB copy calibration), or None."""
+    try:
+        for line in open(PROFILE):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 4 and cells[0] == kernel:
+                return float(cells[3])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def slab_widths(s):
@@ -160,7 +183,7 @@ def main():
     value = n * args.steps / r32["wall_s"] / 1e6
     log(f"[bench] fp32: {value:.0f} Mcell/s, V {r32['kernel_ms_v']:.3f} ms T {r32['kernel_ms_t']:.3f} ms")
     bv, bt, bstep, upd, memb = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"],
-                                                 r32["widths"], 4)
+                                                 r32["widths"], "fp32")
     ach_v = bv / (r32["kernel_ms_v"] * 1e-3) / 1e9
     ach_t = bt / (r32["kernel_ms_t"] * 1e-3) / 1e9
     ach_step = bstep / ((r32["kernel_ms_v"] + r32["kernel_ms_t"]) * 1e-3) / 1e9
@@ -182,9 +205,11 @@ def main():
                                f"(vx, vz, p), FP32 storage",
                    "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "order": 4, "npml": 20,
                    "dx_m": cfg["dx"], "dt_s": cfg["dt"], "tile_bx_by_zchunk": r32["shape"]},
-        "roofline": {"bound": "hbm", "kernel": "k_velocity<float,4>",
+        "roofline": {"bound": "hbm", "kernel": "k_velocity<P32,4>",
                      "achieved": round(ach_v, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach_v / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach_v / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic("k_velocity<P32,4>"),
+                     "traffic_source": "profiles/stencil_kernels_layered256.md",
                      "algorithmic_bytes_per_launch": bv,
                      "mean_launch_ms": round(r32["kernel_ms_v"], 5)},
         "stress_kernel": {"kernel": "k_stress<float,4>", "achieved_GBs": round(ach_t, 1),
@@ -195,20 +220,22 @@ def main():
                           "bytes_per_cell_update": round(bstep / upd, 2)},
         "finite": r32["finite"],
     }
-    if not args.no_fp16:
-        r16 = run_precision(cfg, "fp16", args.steps, args.warmup)
+    for mode in ([] if args.no_fp16 else ["fp16c", "fp16"]):
+        r16 = run_precision(cfg, mode, args.steps, args.warmup)
         v16 = n * args.steps / r16["wall_s"] / 1e6
-        bv16, bt16, bstep16, _, _ = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"],
-                                                      r16["widths"], 2)
+        _, _, bstep16, _, _ = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"],
+                                                r16["widths"], mode)
         a16 = bstep16 / ((r16["kernel_ms_v"] + r16["kernel_ms_t"]) * 1e-3) / 1e9
-        log(f"[bench] fp16: {v16:.0f} Mcell/s ({v16 / value:.2f}x fp32)")
-        out["fp16"] = {"value": round(v16, 1), "unit": "Mcell-updates/s",
-                       "speedup_vs_fp32": round(v16 / value, 3),
-                       "ms_per_step": round(r16["wall_s"] * 1e3 / args.steps, 4),
-                       "step_achieved_GBs": round(a16, 1),
-                       "step_frac": round(a16 / HBM_PEAK_GBS, 4),
-                       "kernel_ms_v": round(r16["kernel_ms_v"], 5),
-                       "kernel_ms_t": round(r16["kernel_ms_t"], 5), "finite": r16["finite"]}
+        log(f"[bench] {mode}: {v16:.0f} Mcell/s ({v16 / value:.2f}x fp32)")
+        out[mode] = {"value": round(v16, 1), "unit": "Mcell-updates/s",
+                     "speedup_vs_fp32": round(v16 / value, 3),
+                     "ms_per_step": round(r16["wall_s"] * 1e3 / args.steps, 4),
+                     "step_achieved_GBs": round(a16, 1),
+                     "step_frac": round(a16 / HBM_PEAK_GBS, 4),
+                     "bytes_per_cell_update": round(bstep16 / upd, 2),
+                     "kernel_ms_v": round(r16["kernel_ms_v"], 5),
+                     "kernel_ms_t": round(r16["kernel_ms_t"], 5), "finite": r16["finite"],
+                     "tile_bx_by_zchunk": r16["shape"]}
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
