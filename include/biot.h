@@ -149,6 +149,46 @@ int biot_last_run_ms(biot_handle *h, double *ms);
  * stencil kernels on the handle's stream; returns their mean durations in ms. */
 int biot_profile_steps(biot_handle *h, int32_t nsteps, double *ms_velocity, double *ms_stress);
 
+/* ---- inversion parameterization and the adjoint (no counterpart in the reference) ---- */
+/* Per-cell inversion parameters (nx*ny*nz doubles each).  lambda (drained Lame), mu, alpha,
+ * M, rho, rho_f are the inverted quantities; C1 = eta/kappa and tphi = C2/rho_f
+ * = (1 + 2/porousm) tao / phi stay fixed.  The update moduli are C = alpha M,
+ * lambda_c = lambda + alpha^2 M, H = lambda_c + 2 mu (outputfunc.cpp:210-217) and the
+ * coefficients follow main.cpp:322-494, evaluated in double. */
+typedef struct biot_params {
+  const double *lambda, *mu, *alpha, *M, *rho, *rhof;
+  const double *C1, *tphi;
+} biot_params;
+int biot_set_params(biot_handle *h, const biot_params *p);
+/* Resets, runs nt steps recording traces (receivers must hold >= nt steps) and keeps the
+ * state every `every` steps on the device for the adjoint.  FP32 and FP64 handles. */
+int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every);
+/* J = 1/2 sum (d - observed)^2 over the recorded traces (the double traces on FP64
+ * handles), reduced in FP64 on the device; residual (may be NULL) receives d - observed in
+ * the trace layout. */
+int biot_misfit_l2(biot_handle *h, const float *observed, float *residual, double *J);
+/* Discrete adjoint of the checkpointed run.  adj_source = dJ/d(trace samples), trace
+ * layout.  Segments of `every` steps are recomputed from their checkpoints and swept
+ * backward; coefficient gradients are summed in FP64 on the device.  grad_params (may be
+ * NULL; needs biot_set_params) receives dJ/d(lambda, mu, alpha, M, rho, rho_f), 6 arrays
+ * of nx*ny*nz; grad_coef (may be NULL) the 22 kernel-coefficient gradients.  The
+ * wavefields are left in an unspecified state. */
+int biot_adjoint(biot_handle *h, const float *adj_source, double *const grad_params[6],
+                 double *const grad_coef[22]);
+/* Order of the 22 kernel coefficient arrays (grad_coef, biot_params_coefficients): the
+ * reference arrays without W3z, with 1/dx and the 0.01 of the stress update folded in. */
+enum biot_kcoef {
+  BIOT_K_U1X = 0, BIOT_K_U1Y, BIOT_K_U1Z, BIOT_K_U2X, BIOT_K_U2Y, BIOT_K_U2Z,
+  BIOT_K_W1X, BIOT_K_W1Y, BIOT_K_W1Z, BIOT_K_W2X, BIOT_K_W2Y, BIOT_K_W2Z, BIOT_K_W3X, BIOT_K_W3Y,
+  BIOT_K_P1, BIOT_K_P2, BIOT_K_S1, BIOT_K_S2, BIOT_K_S3, BIOT_K_SXY, BIOT_K_SXZ, BIOT_K_SYZ,
+  BIOT_NKCOEF
+};
+/* Host only: the kernel coefficients of a parameter set (grid: nx ny nz dx dt Cv Cp), and
+ * the chain rule taking their gradients to dJ/d(lambda, mu, alpha, M, rho, rho_f). */
+int biot_params_coefficients(const biot_grid *g, const biot_params *p, double *const K[22]);
+int biot_params_chain(const biot_grid *g, const biot_params *p, const double *const grad_coef[22],
+                      double *const grad_params[6]);
+
 /* ---- launch shapes ---- */
 /* Tile shape of the two stencil kernels: bx*by threads per block (bx*by <= 256, bx a
  * multiple of 64), each thread marching zchunk planes. */
@@ -162,6 +202,9 @@ int biot_autotune(biot_handle *h, int32_t trial_steps);
 
 /* ---- outputs ---- */
 int biot_read_traces(biot_handle *h, float *out, int64_t count);
+/* FP64 handles also keep the traces in double (the receiver sampling of kernel.cu:2027-2031
+ * without the float store); finite-difference checks of the adjoint read these. */
+int biot_read_traces_f64(biot_handle *h, double *out, int64_t count);
 int biot_read_field(biot_handle *h, int32_t field, float *out);
 int biot_read_field_f64(biot_handle *h, int32_t field, double *out);
 int biot_write_field_f64(biot_handle *h, int32_t field, const double *in);

@@ -56,6 +56,17 @@ class Medium(C.Structure):
                 ("tao", C.c_double), ("porousm", C.c_double)]
 
 
+class Params(C.Structure):
+    _fields_ = [("lambda_", _DP), ("mu", _DP), ("alpha", _DP), ("M", _DP), ("rho", _DP),
+                ("rhof", _DP), ("C1", _DP), ("tphi", _DP)]
+
+
+PARAM_NAMES = ["lambda", "mu", "alpha", "M", "rho", "rhof"]
+FIXED_PARAM_NAMES = ["C1", "tphi"]
+NKCOEF = 22
+KCOEF_NAMES = ["U1X", "U1Y", "U1Z", "U2X", "U2Y", "U2Z", "W1X", "W1Y", "W1Z", "W2X", "W2Y",
+               "W2Z", "W3X", "W3Y", "P1", "P2", "S1", "S2", "S3", "SXY", "SXZ", "SYZ"]
+
 _lib = None
 
 
@@ -98,12 +109,20 @@ def lib():
         "biot_get_launch_shape": ([H, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "biot_autotune": ([H, i32], C.c_int),
         "biot_read_traces": ([H, FP, i64], C.c_int),
+        "biot_read_traces_f64": ([H, _DP, i64], C.c_int),
         "biot_read_field": ([H, i32, FP], C.c_int),
         "biot_read_field_f64": ([H, i32, _DP], C.c_int),
         "biot_write_field_f64": ([H, i32, _DP], C.c_int),
         "biot_snapshot_xz": ([H, i32, i32, FP], C.c_int),
         "biot_get_field_scales": ([H, FP], C.c_int),
         "biot_device_bytes": ([H, C.POINTER(i64)], C.c_int),
+        "biot_set_params": ([H, C.POINTER(Params)], C.c_int),
+        "biot_forward_checkpointed": ([H, i32, i32], C.c_int),
+        "biot_misfit_l2": ([H, FP, FP, _DP], C.c_int),
+        "biot_adjoint": ([H, FP, C.POINTER(_DP), C.POINTER(_DP)], C.c_int),
+        "biot_params_coefficients": ([C.POINTER(Grid), C.POINTER(Params), C.POINTER(_DP)], C.c_int),
+        "biot_params_chain": ([C.POINTER(Grid), C.POINTER(Params), C.POINTER(_DP), C.POINTER(_DP)],
+                              C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -122,9 +141,11 @@ def exported_symbols():
             "biot_set_receivers", "biot_reset", "biot_run", "biot_step_count",
             "biot_synchronize", "biot_last_run_ms", "biot_profile_steps",
             "biot_set_launch_shape", "biot_get_launch_shape", "biot_autotune",
-            "biot_read_traces", "biot_read_field",
+            "biot_read_traces", "biot_read_traces_f64", "biot_read_field",
             "biot_read_field_f64", "biot_write_field_f64", "biot_snapshot_xz",
-            "biot_get_field_scales", "biot_device_bytes"]
+            "biot_get_field_scales", "biot_device_bytes", "biot_set_params",
+            "biot_forward_checkpointed", "biot_misfit_l2", "biot_adjoint",
+            "biot_params_coefficients", "biot_params_chain"]
 
 
 def fptr(a):
@@ -174,3 +195,50 @@ def pml_profile(n, dx, dt, npml, v0=4000.0, R=1e-6):
     if rc != BIOT_OK:
         raise BiotError(rc, "biot_pml_profile_1d")
     return arrs
+
+
+def params_struct(params, n):
+    """Params structure over a dict of the eight per-cell arrays (kept alive by the caller
+    through the returned tuple's second element)."""
+    keep = {k: np.ascontiguousarray(np.asarray(params[k], np.float64).reshape(-1))
+            for k in PARAM_NAMES + FIXED_PARAM_NAMES}
+    for k, a in keep.items():
+        if a.size != n:
+            raise ValueError(f"parameter {k} has {a.size} cells, expected {n}")
+    st = Params(lambda_=dptr(keep["lambda"]), mu=dptr(keep["mu"]), alpha=dptr(keep["alpha"]),
+                M=dptr(keep["M"]), rho=dptr(keep["rho"]), rhof=dptr(keep["rhof"]),
+                C1=dptr(keep["C1"]), tphi=dptr(keep["tphi"]))
+    return st, keep
+
+
+def _host_grid(nx, ny, nz, dx, dt, Cv, Cp):
+    g = Grid()
+    g.nx, g.ny, g.nz, g.dx, g.dt, g.Cv, g.Cp = nx, ny, nz, dx, dt, Cv, Cp
+    return g
+
+
+def params_coefficients(nx, ny, nz, dx, dt, params, Cv=1e8, Cp=1.0):
+    """Host-only: the 22 kernel coefficient arrays (double, kernel order) of a parameter set."""
+    n = nx * ny * nz
+    st, keep = params_struct(params, n)
+    out = [np.zeros(n, np.float64) for _ in range(NKCOEF)]
+    ptrs = (_DP * NKCOEF)(*[dptr(a) for a in out])
+    g = _host_grid(nx, ny, nz, dx, dt, Cv, Cp)
+    rc = lib().biot_params_coefficients(C.byref(g), C.byref(st), ptrs)
+    if rc != BIOT_OK:
+        raise BiotError(rc, "biot_params_coefficients")
+    return out
+
+
+def params_chain(nx, ny, nz, dx, dt, params, grad_coef, Cv=1e8, Cp=1.0):
+    """Host-only: dJ/dK (22 arrays) -> dJ/d(lambda, mu, alpha, M, rho, rho_f)."""
+    n = nx * ny * nz
+    st, keep = params_struct(params, n)
+    gk = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1)) for a in grad_coef]
+    out = [np.zeros(n, np.float64) for _ in range(6)]
+    g = _host_grid(nx, ny, nz, dx, dt, Cv, Cp)
+    rc = lib().biot_params_chain(C.byref(g), C.byref(st), (_DP * NKCOEF)(*[dptr(a) for a in gk]),
+                                 (_DP * 6)(*[dptr(a) for a in out]))
+    if rc != BIOT_OK:
+        raise BiotError(rc, "biot_params_chain")
+    return out

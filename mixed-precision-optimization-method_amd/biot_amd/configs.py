@@ -51,6 +51,28 @@ def perturbed512(order=4, nt=3000, nx=512, ny=512, nz=384, seed=2024, stride=4):
                 rec_fields=REC_FIELDS)
 
 
+def adjoint128(order=4, nt=1000, n=128, every=50, sphere=True):
+    """True model: homogeneous rock with a +8 % Kb sphere of radius 30 m (6 cells) at the
+    centre; the start model has no sphere (make it with sphere=False).  Explosive 25 Hz
+    source above the sphere, receivers every 8 cells on the planes z = 16 and z = n - 16
+    recording vx, vz, p."""
+    shape = (n, n, n)
+    npml = 10
+    m = media.homogeneous(shape)
+    if sphere:
+        c = n // 2
+        kk, jj, ii = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        ball = (ii - c) ** 2 + (jj - c) ** 2 + (kk - c) ** 2 <= 6 ** 2
+        m["Kb"] = np.where(ball, m["Kb"] * 1.08, m["Kb"])
+    g = np.arange(4, n - 3, 8)
+    ii, jj = np.meshgrid(g, g, indexing="xy")
+    rec = np.concatenate([np.stack([ii.ravel(), jj.ravel(), np.full(ii.size, z)], axis=1)
+                          for z in (16, n - 16)])
+    return dict(name="adjoint128", nx=n, ny=n, nz=n, dx=DX, dt=DT, order=order, npml=npml,
+                nt=nt, every=every, f0=25.0, src=(n // 2, n // 2, 24), src_fields=SRC_EXPLOSIVE,
+                medium=m, rec=rec, rec_fields=REC_FIELDS)
+
+
 def cells(cfg, ijk):
     ijk = np.asarray(ijk, dtype=np.int64).reshape(-1, 3)
     return ijk[:, 0] + ijk[:, 1] * cfg["nx"] + ijk[:, 2] * cfg["nx"] * cfg["ny"]

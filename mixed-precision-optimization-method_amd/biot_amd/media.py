@@ -60,3 +60,28 @@ def perturbed(shape, amplitude=0.10, corr_cells=10.0, seed=2024, keys=("Kb", "mu
 def crop(medium, k0, j0, i0, size):
     return {k: np.ascontiguousarray(v[k0:k0 + size[0], j0:j0 + size[1], i0:i0 + size[2]])
             for k, v in medium.items()}
+
+
+def params_from_medium(medium, tao=3.0, porousm=8.0):
+    """Inversion parameters of a medium dict, in double: drained Lame lambda = Kb - 2 mu / 3,
+    mu, alpha = 1 - Kb/Ks, M = Kf Ks / (phi Ks + (alpha - phi) Kf), rho = (1 - phi) rho_s +
+    phi rho_f, rho_f, and the fixed C1 = eta/perm, tphi = (1 + 2/porousm) tao / phi
+    (outputfunc.cpp:205-217).  Fluid cells (mu == 0 and Kb == 0, outputfunc.cpp:176-193)
+    map to lambda = mu = 0, alpha = 1, M = Kf, rho = rho_f, C1 = 0, tphi = 1."""
+    g = {k: np.asarray(v, np.float64) for k, v in medium.items()}
+    shape = np.broadcast_shapes(*[v.shape for v in g.values()])
+    g = {k: np.broadcast_to(v, shape) for k, v in g.items()}
+    fluid = (g["mu"] == 0) & (g["Kb"] == 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        alpha = 1.0 - g["Kb"] / g["Ks"]
+        M = g["Kf"] * g["Ks"] / (g["phi"] * g["Ks"] + (alpha - g["phi"]) * g["Kf"])
+        out = dict(lam=g["Kb"] - 2.0 * g["mu"] / 3.0, mu=g["mu"], alpha=alpha, M=M,
+                   rho=(1 - g["phi"]) * g["rhos"] + g["phi"] * g["rhof"], rhof=g["rhof"],
+                   C1=g["eta"] / g["perm"], tphi=(1 + 2 / porousm) * tao / g["phi"])
+    fl = dict(lam=0.0, mu=0.0, alpha=1.0, M=g["Kf"], rho=g["rhof"], rhof=g["rhof"], C1=0.0,
+              tphi=1.0)
+    res = {}
+    for k, v in out.items():
+        f = np.broadcast_to(np.asarray(fl[k], np.float64), shape)
+        res["lambda" if k == "lam" else k] = np.where(fluid, f, v).astype(np.float64)
+    return res

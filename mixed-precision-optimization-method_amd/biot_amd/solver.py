@@ -185,3 +185,50 @@ class Solver:
         b = C.c_int64()
         self._check(lib().biot_device_bytes(self._h, C.byref(b)), "biot_device_bytes")
         return b.value
+
+    def traces_f64(self):
+        """Receiver traces in double (FP64 handles only), shape (ncomp, nt_max, nrec)."""
+        out = np.zeros((self.ncomp, self.nt_max, self.nrec), np.float64)
+        self._check(lib().biot_read_traces_f64(self._h, dptr(out), out.size), "biot_read_traces_f64")
+        return out
+
+    # -- inversion -----------------------------------------------------------------------
+    def set_params(self, params):
+        """Per-cell inversion parameters: dict with lambda mu alpha M rho rhof (inverted) and
+        C1 tphi (fixed), arrays of shape (nz, ny, nx) or scalars."""
+        full = {k: np.broadcast_to(np.asarray(params[k], np.float64), self.shape)
+                for k in _lib.PARAM_NAMES + _lib.FIXED_PARAM_NAMES}
+        st, keep = _lib.params_struct(full, self.n)
+        self._check(lib().biot_set_params(self._h, C.byref(st)), "biot_set_params")
+
+    def forward_checkpointed(self, nt, every=50):
+        """Reset, nt recorded steps, a device checkpoint every `every` steps."""
+        self._check(lib().biot_forward_checkpointed(self._h, int(nt), int(every)),
+                    "biot_forward_checkpointed")
+
+    def misfit(self, observed):
+        """(J, residual): J = 1/2 sum (d - observed)^2 reduced in FP64 on the device."""
+        obs = np.ascontiguousarray(np.asarray(observed, np.float32))
+        if obs.shape != (self.ncomp, self.nt_max, self.nrec):
+            raise ValueError(f"observed traces {obs.shape}, expected "
+                             f"{(self.ncomp, self.nt_max, self.nrec)}")
+        res = np.zeros_like(obs)
+        J = C.c_double()
+        self._check(lib().biot_misfit_l2(self._h, fptr(obs), fptr(res), C.byref(J)), "biot_misfit_l2")
+        return J.value, res
+
+    def adjoint(self, adj_source, params=True, coefs=False):
+        """Adjoint of the last checkpointed run for the trace-space gradient adj_source.
+        Returns (dict of dJ/d parameter arrays or None, list of 22 dJ/dK arrays or None)."""
+        a = np.ascontiguousarray(np.asarray(adj_source, np.float32))
+        if a.shape != (self.ncomp, self.nt_max, self.nrec):
+            raise ValueError(f"adjoint source {a.shape}, expected "
+                             f"{(self.ncomp, self.nt_max, self.nrec)}")
+        gp = [np.zeros(self.n, np.float64) for _ in range(6)] if params else None
+        gk = [np.zeros(self.n, np.float64) for _ in range(_lib.NKCOEF)] if coefs else None
+        pp = (C.POINTER(C.c_double) * 6)(*[dptr(x) for x in gp]) if params else None
+        kp = (C.POINTER(C.c_double) * _lib.NKCOEF)(*[dptr(x) for x in gk]) if coefs else None
+        self._check(lib().biot_adjoint(self._h, fptr(a), pp, kp), "biot_adjoint")
+        outp = {k: v.reshape(self.shape) for k, v in zip(_lib.PARAM_NAMES, gp)} if params else None
+        outk = [v.reshape(self.shape) for v in gk] if coefs else None
+        return outp, outk

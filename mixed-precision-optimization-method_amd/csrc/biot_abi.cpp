@@ -254,6 +254,7 @@ extern "C" int biot_set_medium(biot_handle *h, const biot_medium *m)
                    !(m->eta[c] >= 0) || !(m->mu[c] >= 0) || !(m->Kb[c] >= 0)))
       return fail(h, BIOT_ERR_ARG, "cell %lld: invalid porous-medium parameters", (long long)c);
   }
+  h->forget_model();
   Moduli md;
   Averages av;
   build_moduli(h->n, *m, md);
@@ -291,8 +292,10 @@ extern "C" int biot_set_coefficients(biot_handle *h, const float *const coef[BIO
 {
   if (!h) return BIOT_ERR_ARG;
   if (!coef) return fail(h, BIOT_ERR_ARG, "coef is null");
-  for (int a = 0; a < BIOT_NCOEF; ++a) {
+  for (int a = 0; a < BIOT_NCOEF; ++a)
     if (!coef[a] && a != BIOT_W3Z) return fail(h, BIOT_ERR_ARG, "coefficient array %d is null", a);
+  h->forget_model();
+  for (int a = 0; a < BIOT_NCOEF; ++a) {
     if (coef[a])
       h->coef_ref[a].assign(coef[a], coef[a] + h->n);
     else
@@ -305,6 +308,8 @@ extern "C" int biot_get_coefficients(biot_handle *h, float *const coef[BIOT_NCOE
 {
   if (!h || !coef) return BIOT_ERR_ARG;
   if (!h->have_coef) return fail(h, BIOT_ERR_STATE, "no coefficients set");
+  if (h->coef_ref[0].empty())
+    return fail(h, BIOT_ERR_STATE, "coefficients built by biot_set_params have no reference-representation copy");
   for (int a = 0; a < BIOT_NCOEF; ++a)
     if (coef[a]) std::memcpy(coef[a], h->coef_ref[a].data(), (size_t)h->n * sizeof(float));
   return BIOT_OK;
@@ -338,6 +343,7 @@ extern "C" int biot_set_source(biot_handle *h, int64_t cell, uint32_t mask, cons
                                int32_t nt)
 {
   if (!h) return BIOT_ERR_ARG;
+  h->free_adjoint();  // checkpoints of another source are stale
   if (cell < 0 || cell >= h->n) return fail(h, BIOT_ERR_ARG, "source cell %lld outside grid", (long long)cell);
   if (mask >> NFIELD) return fail(h, BIOT_ERR_ARG, "source field mask has bits above field 12");
   if (nt < 0 || (nt > 0 && !w)) return fail(h, BIOT_ERR_ARG, "wavelet null or nt < 0");
@@ -358,6 +364,7 @@ extern "C" int biot_set_receivers(biot_handle *h, const int64_t *cells, int64_t 
                                   uint32_t mask, int32_t nt_max)
 {
   if (!h) return BIOT_ERR_ARG;
+  h->free_adjoint();  // the adjoint source layout follows the receivers
   if (nrec < 0 || (nrec > 0 && !cells)) return fail(h, BIOT_ERR_ARG, "receiver list invalid");
   if (mask >> NFIELD) return fail(h, BIOT_ERR_ARG, "receiver field mask has bits above field 12");
   if (nt_max < 0) return fail(h, BIOT_ERR_ARG, "nt_max < 0");
@@ -367,8 +374,10 @@ extern "C" int biot_set_receivers(biot_handle *h, const int64_t *cells, int64_t 
                   (long long)cells[r]);
   if (h->d_rec) (void)hipFree(h->d_rec);
   if (h->d_traces) (void)hipFree(h->d_traces);
+  if (h->d_traces64) (void)hipFree(h->d_traces64);
   h->d_rec = nullptr;
   h->d_traces = nullptr;
+  h->d_traces64 = nullptr;
   h->ncomp = 0;
   for (int f = 0; f < NFIELD; ++f)
     if (mask & (1u << f)) h->comp[h->ncomp++] = f;
@@ -383,6 +392,10 @@ extern "C" int biot_set_receivers(biot_handle *h, const int64_t *cells, int64_t 
   if (tr > 0) {
     HIPCHK(h, hipMalloc(&h->d_traces, tr * sizeof(float)));
     HIPCHK(h, hipMemsetAsync(h->d_traces, 0, tr * sizeof(float), h->stream));
+    if (h->prec == BIOT_FP64) {
+      HIPCHK(h, hipMalloc((void **)&h->d_traces64, tr * sizeof(double)));
+      HIPCHK(h, hipMemsetAsync(h->d_traces64, 0, tr * sizeof(double), h->stream));
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
   }
   return BIOT_OK;
@@ -402,6 +415,9 @@ extern "C" int biot_reset(biot_handle *h)
   if (h->d_traces)
     HIPCHK(h, hipMemsetAsync(h->d_traces, 0, (size_t)h->ncomp * h->nt_max * h->nrec * sizeof(float),
                              h->stream));
+  if (h->d_traces64)
+    HIPCHK(h, hipMemsetAsync(h->d_traces64, 0,
+                             (size_t)h->ncomp * h->nt_max * h->nrec * sizeof(double), h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->it = 0;
   return BIOT_OK;
@@ -431,6 +447,7 @@ int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks)
     ra.scale[c] = fscale[comp[c]];
   }
   ra.traces = (float *)d_traces;
+  ra.traces64 = d_traces64;
   ra.nt_max = nt_max;
 
   StepArgs args;
@@ -677,6 +694,20 @@ extern "C" int biot_read_traces(biot_handle *h, float *out, int64_t count)
   if (need == 0) return BIOT_OK;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(out, h->d_traces, (size_t)need * sizeof(float), hipMemcpyDeviceToHost));
+  return BIOT_OK;
+}
+
+extern "C" int biot_read_traces_f64(biot_handle *h, double *out, int64_t count)
+{
+  if (!h || !out) return BIOT_ERR_ARG;
+  if (h->prec != BIOT_FP64) return fail(h, BIOT_ERR_STATE, "double traces are kept by FP64 handles only");
+  const int64_t need = (int64_t)h->ncomp * h->nt_max * h->nrec;
+  if (count < need)
+    return fail(h, BIOT_ERR_ARG, "trace buffer holds %lld, need %lld", (long long)count,
+                (long long)need);
+  if (need == 0) return BIOT_OK;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(out, h->d_traces64, (size_t)need * sizeof(double), hipMemcpyDeviceToHost));
   return BIOT_OK;
 }
 

@@ -1,0 +1,326 @@
+// C-ABI of the inversion parameterization, the checkpointed forward run, the FP64 misfit
+// and the discrete adjoint (include/biot.h).  None of this exists in the reference, whose
+// driver is forward-only (kernel.cu:1728-2697).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/biot.h"
+#include "biot_adjoint.hpp"
+#include "biot_handle.hpp"
+#include "biot_params.hpp"
+
+using namespace biot;
+
+namespace {
+
+int afail(biot_handle *h, int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  h->err = buf;
+  return code;
+}
+
+#define AHIP(h, call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return afail((h), BIOT_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
+  } while (0)
+
+ParamArrays param_view(const biot_handle *h)
+{
+  const auto &P = h->par;
+  return ParamArrays{P[0].data(), P[1].data(), P[2].data(), P[3].data(),
+                     P[4].data(), P[5].data(), P[6].data(), P[7].data()};
+}
+
+CoefScales coef_scales(const biot_handle *h)
+{
+  return CoefScales{(double)h->g.dt, (double)h->g.Cv, (double)h->g.Cp, (double)(1.0f / h->g.dx)};
+}
+
+size_t arith_bytes(int prec) { return prec == BIOT_FP64 ? 8 : 4; }
+
+}  // namespace
+
+extern "C" int biot_set_params(biot_handle *h, const biot_params *p)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (!p || !p->lambda || !p->mu || !p->alpha || !p->M || !p->rho || !p->rhof || !p->C1 || !p->tphi)
+    return afail(h, BIOT_ERR_ARG, "biot_set_params: every per-cell array must be given");
+  const double *src[8] = {p->lambda, p->mu, p->alpha, p->M, p->rho, p->rhof, p->C1, p->tphi};
+  for (int64_t c = 0; c < h->n; ++c) {
+    if (!(p->rho[c] > 0) || !(p->rhof[c] > 0) || !(p->M[c] > 0) || !(p->mu[c] >= 0) ||
+        !(p->C1[c] >= 0) || !(p->tphi[c] > 0))
+      return afail(h, BIOT_ERR_ARG, "cell %lld: invalid parameters", (long long)c);
+  }
+  h->free_adjoint();  // checkpoints of the previous model are stale
+  for (int q = 0; q < 8; ++q) h->par[q].assign(src[q], src[q] + h->n);
+  kernel_coefs_from_params(h->g.nx, h->g.ny, h->g.nz, coef_scales(h), param_view(h), h->K_last);
+  for (auto &v : h->coef_ref) std::vector<float>().swap(v);  // the reference arrays no longer apply
+  return h->upload_kernel_coefs(h->K_last);
+}
+
+extern "C" int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every)
+{
+  if (!h) return BIOT_ERR_ARG;
+  if (h->prec != BIOT_FP32 && h->prec != BIOT_FP64)
+    return afail(h, BIOT_ERR_ARG, "the adjoint path runs on FP32 or FP64 handles");
+  if (nt <= 0 || every <= 0) return afail(h, BIOT_ERR_ARG, "nt and every must be positive");
+  if (!h->have_coef) return afail(h, BIOT_ERR_STATE, "no medium set");
+  if (h->nt_max < nt || h->ncomp == 0) return afail(h, BIOT_ERR_STATE, "receivers must record >= nt steps");
+  for (uint32_t f = 0; f < NFIELD; ++f)
+    if ((h->src_mask >> f) & 1u && !(f >= 6))
+      return afail(h, BIOT_ERR_ARG, "the adjoint path supports sources on stresses and pressure only");
+  h->free_adjoint();
+  const int nseg = (nt + every - 1) / every;
+  const size_t sb = h->state_bytes();
+  for (int s = 0; s < nseg; ++s) {
+    void *p = nullptr;
+    if (hipMalloc(&p, sb) != hipSuccess) {
+      h->free_adjoint();
+      return afail(h, BIOT_ERR_NOMEM, "checkpoint %d of %d (%zu bytes) does not fit", s, nseg, sb);
+    }
+    h->ckpt.push_back(p);
+  }
+  int rc = biot_reset(h);
+  if (rc) return rc;
+  AHIP(h, hipEventRecord(h->ev0, h->stream));
+  for (int s = 0; s < nseg; ++s) {
+    h->save_state(h->ckpt[s]);
+    rc = h->enqueue_steps(std::min(every, nt - s * every), nullptr);
+    if (rc) return rc;
+  }
+  AHIP(h, hipEventRecord(h->ev1, h->stream));
+  AHIP(h, hipEventSynchronize(h->ev1));
+  float ms = 0;
+  AHIP(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->ckpt_every = every;
+  h->ckpt_nt = nt;
+  return BIOT_OK;
+}
+
+extern "C" int biot_misfit_l2(biot_handle *h, const float *observed, float *residual, double *J)
+{
+  if (!h || !observed || !J) return BIOT_ERR_ARG;
+  const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
+  if (m == 0) return afail(h, BIOT_ERR_STATE, "no receivers");
+  float *d_obs = nullptr, *d_res = nullptr;
+  double *d_J = nullptr;
+  int rc = BIOT_OK;
+  if (hipMalloc(&d_obs, m * sizeof(float)) != hipSuccess || hipMalloc(&d_res, m * sizeof(float)) != hipSuccess ||
+      hipMalloc(&d_J, sizeof(double)) != hipSuccess)
+    rc = afail(h, BIOT_ERR_NOMEM, "misfit buffers");
+  if (!rc && (hipMemcpyAsync(d_obs, observed, m * sizeof(float), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+              hipMemsetAsync(d_J, 0, sizeof(double), h->stream) != hipSuccess ||
+              launch_misfit(h->d_traces64 ? (const void *)h->d_traces64 : h->d_traces, h->d_traces64 != nullptr,
+                            d_obs, d_res, m, d_J, h->stream) != hipSuccess ||
+              hipMemcpyAsync(J, d_J, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+              (residual && hipMemcpyAsync(residual, d_res, m * sizeof(float), hipMemcpyDeviceToHost,
+                                          h->stream) != hipSuccess) ||
+              hipStreamSynchronize(h->stream) != hipSuccess))
+    rc = afail(h, BIOT_ERR_HIP, "misfit reduction failed");
+  for (void *p : {(void *)d_obs, (void *)d_res, (void *)d_J})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+extern "C" int biot_adjoint(biot_handle *h, const float *adj_source, double *const grad_params[6],
+                            double *const grad_coef[22])
+{
+  if (!h || !adj_source) return BIOT_ERR_ARG;
+  if (h->ckpt_nt <= 0) return afail(h, BIOT_ERR_STATE, "biot_adjoint needs biot_forward_checkpointed first");
+  if (grad_params && h->par[0].empty())
+    return afail(h, BIOT_ERR_STATE, "parameter gradients need biot_set_params");
+  const int nt = h->ckpt_nt, every = h->ckpt_every, order = h->g.order, hh = order / 2;
+  const int64_t n = h->n;
+  const size_t asz = arith_bytes(h->prec), sb = h->state_bytes();
+  // ---- buffers
+  size_t lam_bytes = (size_t)NFIELD * n * asz + (size_t)12 * n * asz;
+  for (int ax = 0; ax < 3; ++ax) lam_bytes += (size_t)NCARRY * h->slab[ax] * asz;
+  const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
+  if (!h->d_slots && hipMalloc(&h->d_slots, sb * (size_t)(every + 1)) != hipSuccess) {
+    h->d_slots = nullptr;
+    return afail(h, BIOT_ERR_NOMEM, "segment buffer (%d states)", every + 1);
+  }
+  if (!h->d_lam && hipMalloc(&h->d_lam, lam_bytes) != hipSuccess) {
+    h->d_lam = nullptr;
+    return afail(h, BIOT_ERR_NOMEM, "adjoint state");
+  }
+  if (!h->d_grad && hipMalloc((void **)&h->d_grad, (size_t)NKC * n * sizeof(double)) != hipSuccess) {
+    h->d_grad = nullptr;
+    return afail(h, BIOT_ERR_NOMEM, "gradient sums");
+  }
+  if (!h->d_adjsrc && hipMalloc((void **)&h->d_adjsrc, (size_t)m * sizeof(float)) != hipSuccess) {
+    h->d_adjsrc = nullptr;
+    return afail(h, BIOT_ERR_NOMEM, "adjoint source");
+  }
+  AHIP(h, hipMemsetAsync(h->d_lam, 0, lam_bytes, h->stream));
+  AHIP(h, hipMemsetAsync(h->d_grad, 0, (size_t)NKC * n * sizeof(double), h->stream));
+  AHIP(h, hipMemcpyAsync(h->d_adjsrc, adj_source, (size_t)m * sizeof(float), hipMemcpyHostToDevice, h->stream));
+
+  // ---- argument blocks
+  AdjArgs A;
+  std::memset(&A, 0, sizeof A);
+  char *lp = (char *)h->d_lam;
+  for (int f = 0; f < NFIELD; ++f) A.lam[f] = lp + (size_t)f * n * asz;
+  lp += (size_t)NFIELD * n * asz;
+  for (int ax = 0; ax < 3; ++ax) {
+    A.lcarry[ax] = lp;
+    lp += (size_t)NCARRY * h->slab[ax] * asz;
+  }
+  for (int d = 0; d < 12; ++d) A.G[d] = lp + (size_t)d * n * asz;
+  for (int k = 0; k < NKC; ++k) {
+    A.grad[k] = h->d_grad + (size_t)k * n;
+    A.k[k] = (const char *)h->d_coef + (size_t)k * n * h->esz;
+  }
+  for (int ax = 0; ax < 3; ++ax) {
+    for (int q = 0; q < 4; ++q) A.prof[ax][q] = h->d_prof[ax][q];
+    A.L[ax] = h->L[ax];
+    A.R[ax] = h->R[ax];
+    A.slab[ax] = h->slab[ax];
+  }
+  A.nx = h->g.nx;
+  A.ny = h->g.ny;
+  A.nz = h->g.nz;
+  A.sy = h->g.nx;
+  A.sz = (int64_t)h->g.nx * h->g.ny;
+  A.nupd = (uint32_t)((int64_t)(h->g.nx - 2 * hh) * (h->g.ny - 2 * hh) * (h->g.nz - 2 * hh));
+  auto slot = [&](int s) { return (char *)h->d_slots + (size_t)s * sb; };
+  auto point_at = [&](int s, const void **fields, const void **carry) {
+    char *p = slot(s);
+    for (int f = 0; f < NFIELD; ++f) fields[f] = p + (size_t)f * n * h->esz;
+    p += (size_t)NFIELD * n * h->esz;
+    for (int ax = 0; ax < 3; ++ax) {
+      if (carry) carry[ax] = p;
+      p += (size_t)NCARRY * h->slab[ax] * h->csz;
+    }
+  };
+  InjectArgs I;
+  std::memset(&I, 0, sizeof I);
+  for (int f = 0; f < NFIELD; ++f) I.lam[f] = A.lam[f];
+  I.rec = (const int64_t *)h->d_rec;
+  I.nrec = h->nrec;
+  I.ncomp = h->ncomp;
+  for (int c = 0; c < h->ncomp; ++c) {
+    I.comp[c] = h->comp[c];
+    I.scale[c] = h->fscale[h->comp[c]];  // traces hold scale x stored value
+  }
+  I.src = h->d_adjsrc;
+  I.nt = h->nt_max;
+  SourceArgs sa;
+  std::memset(&sa, 0, sizeof sa);
+  for (int f = 0; f < NFIELD; ++f) {
+    sa.f[f] = h->field_ptr(f);
+    sa.lo[f] = h->lo_ptr(f);
+    sa.inv_scale[f] = 1.0f / h->fscale[f];
+  }
+  sa.cell = h->src_cell;
+  sa.mask = h->src_mask;
+  sa.wavelet = (const float *)h->d_wavelet;
+  StepArgs st;
+  h->fill_args(st);
+
+  // ---- backward sweep over the checkpoint segments
+  AHIP(h, hipEventRecord(h->ev0, h->stream));
+  const int nseg = (int)h->ckpt.size();
+  for (int s = nseg - 1; s >= 0; --s) {
+    const int n0 = s * every, K = std::min(every, nt - n0);
+    h->load_state(h->ckpt[s]);
+    for (int q = 0; q < K; ++q) {
+      const int it = n0 + q;
+      if (h->d_wavelet && h->src_mask && it < (int)h->wavelet.size()) {
+        sa.it = it;
+        AHIP(h, launch_source(h->prec, sa, h->stream));
+      }
+      h->save_state(slot(q));  // state of step it after its source
+      AHIP(h, launch_velocity(h->prec, order, st, h->shape, h->stream));
+      AHIP(h, launch_stress(h->prec, order, st, h->shape, h->stream));
+    }
+    h->save_state(slot(K));
+    for (int it = n0 + K - 1; it >= n0; --it) {
+      I.it = it;
+      AHIP(h, launch_adjoint_inject(h->prec, I, h->stream));
+      point_at(it - n0, A.fw, A.fcarry);
+      point_at(it - n0 + 1, A.fw1, nullptr);
+      AHIP(h, launch_adjoint_step(h->prec, order, A, h->stream));
+    }
+  }
+  AHIP(h, hipEventRecord(h->ev1, h->stream));
+  AHIP(h, hipEventSynchronize(h->ev1));
+  float ms = 0;
+  AHIP(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+
+  // ---- gradients out
+  std::vector<double> gK((size_t)NKC * n);
+  AHIP(h, hipMemcpy(gK.data(), h->d_grad, gK.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (grad_coef)
+    for (int k = 0; k < NKC; ++k)
+      if (grad_coef[k]) std::memcpy(grad_coef[k], gK.data() + (size_t)k * n, (size_t)n * sizeof(double));
+  if (grad_params) {
+    const double *gk[NKC];
+    for (int k = 0; k < NKC; ++k) gk[k] = gK.data() + (size_t)k * n;
+    for (int q = 0; q < 6; ++q)
+      if (!grad_params[q]) return afail(h, BIOT_ERR_ARG, "grad_params[%d] is null", q);
+    for (int q = 0; q < 6; ++q) std::memset(grad_params[q], 0, (size_t)n * sizeof(double));
+    chain_params(h->g.nx, h->g.ny, h->g.nz, coef_scales(h), param_view(h), gk, grad_params);
+  }
+  return BIOT_OK;
+}
+
+static_assert((int)BIOT_K_SYZ == (int)K_SYZ && (int)BIOT_NKCOEF == (int)NKC, "kernel coefficient order");
+
+namespace {
+bool host_grid_ok(const biot_grid *g)
+{
+  return g && g->nx >= 3 && g->ny >= 3 && g->nz >= 3 && g->dx > 0 && g->dt > 0 && g->Cv > 0 && g->Cp > 0;
+}
+bool params_ok(const biot_params *p)
+{
+  return p && p->lambda && p->mu && p->alpha && p->M && p->rho && p->rhof && p->C1 && p->tphi;
+}
+CoefScales grid_scales(const biot_grid *g)
+{
+  return CoefScales{(double)g->dt, (double)g->Cv, (double)g->Cp, (double)(1.0f / g->dx)};
+}
+ParamArrays param_arrays(const biot_params *p)
+{
+  return ParamArrays{p->lambda, p->mu, p->alpha, p->M, p->rho, p->rhof, p->C1, p->tphi};
+}
+}  // namespace
+
+extern "C" int biot_params_coefficients(const biot_grid *g, const biot_params *p, double *const K[22])
+{
+  if (!host_grid_ok(g) || !params_ok(p) || !K) return BIOT_ERR_ARG;
+  for (int k = 0; k < NKC; ++k)
+    if (!K[k]) return BIOT_ERR_ARG;
+  std::vector<double> Kv[NKC];
+  kernel_coefs_from_params(g->nx, g->ny, g->nz, grid_scales(g), param_arrays(p), Kv);
+  for (int k = 0; k < NKC; ++k) std::memcpy(K[k], Kv[k].data(), Kv[k].size() * sizeof(double));
+  return BIOT_OK;
+}
+
+extern "C" int biot_params_chain(const biot_grid *g, const biot_params *p, const double *const grad_coef[22],
+                                 double *const grad_params[6])
+{
+  if (!host_grid_ok(g) || !params_ok(p) || !grad_coef || !grad_params) return BIOT_ERR_ARG;
+  for (int k = 0; k < NKC; ++k)
+    if (!grad_coef[k]) return BIOT_ERR_ARG;
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  for (int q = 0; q < 6; ++q) {
+    if (!grad_params[q]) return BIOT_ERR_ARG;
+    std::memset(grad_params[q], 0, n * sizeof(double));
+  }
+  chain_params(g->nx, g->ny, g->nz, grid_scales(g), param_arrays(p), grad_coef, grad_params);
+  return BIOT_OK;
+}

@@ -87,6 +87,7 @@ struct RecordArgs {
   int ncomp;
   float scale[NFIELD];      // storage scale per recorded component
   float *traces;            // device [comp][nt_max][nrec]
+  double *traces64;         // same layout in double (FP64 handles), or null
   int it, nt_max;
 };
 hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s);

@@ -8,8 +8,62 @@
 
 using namespace biot;
 
+size_t biot_handle::state_bytes() const
+{
+  size_t b = (size_t)NFIELD * n * esz;
+  for (int ax = 0; ax < 3; ++ax) b += (size_t)NCARRY * slab[ax] * csz;
+  return b;
+}
+
+// the whole time-stepping state (wavefields + C-PML carries) to / from one device buffer
+void biot_handle::save_state(void *dst)
+{
+  char *p = (char *)dst;
+  (void)hipMemcpyAsync(p, d_fields, (size_t)NFIELD * n * esz, hipMemcpyDeviceToDevice, stream);
+  p += (size_t)NFIELD * n * esz;
+  for (int ax = 0; ax < 3; ++ax) {
+    const size_t b = (size_t)NCARRY * slab[ax] * csz;
+    if (b) (void)hipMemcpyAsync(p, d_carry[ax], b, hipMemcpyDeviceToDevice, stream);
+    p += b;
+  }
+}
+
+void biot_handle::load_state(const void *src)
+{
+  const char *p = (const char *)src;
+  (void)hipMemcpyAsync(d_fields, p, (size_t)NFIELD * n * esz, hipMemcpyDeviceToDevice, stream);
+  p += (size_t)NFIELD * n * esz;
+  for (int ax = 0; ax < 3; ++ax) {
+    const size_t b = (size_t)NCARRY * slab[ax] * csz;
+    if (b) (void)hipMemcpyAsync(d_carry[ax], p, b, hipMemcpyDeviceToDevice, stream);
+    p += b;
+  }
+}
+
+void biot_handle::free_adjoint()
+{
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (void *p : ckpt)
+    if (p) (void)hipFree(p);
+  ckpt.clear();
+  ckpt_every = ckpt_nt = 0;
+  for (void **p : {&d_slots, &d_lam, (void **)&d_grad, (void **)&d_adjsrc})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+}
+
+void biot_handle::forget_model()
+{
+  free_adjoint();
+  for (auto &v : par) std::vector<double>().swap(v);
+  for (auto &v : K_last) std::vector<double>().swap(v);
+}
+
 biot_handle::~biot_handle()
 {
+  free_adjoint();
   if (stream) (void)hipStreamSynchronize(stream);
   if (d_fields) (void)hipFree(d_fields);
   if (d_lo) (void)hipFree(d_lo);
@@ -22,6 +76,7 @@ biot_handle::~biot_handle()
   if (d_wavelet) (void)hipFree(d_wavelet);
   if (d_rec) (void)hipFree(d_rec);
   if (d_traces) (void)hipFree(d_traces);
+  if (d_traces64) (void)hipFree(d_traces64);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -178,19 +233,32 @@ double amax(const std::vector<double> &v)
 }  // namespace
 
 // Builds the 22 kernel coefficient arrays from the reference arrays and uploads them.
+int biot_handle::upload_coefficients()
+{
+  if (coef_ref[0].empty()) {
+    if (!K_last[0].empty()) return upload_kernel_coefs(K_last);  // parameter-built set
+    err = "no coefficients to upload";
+    return BIOT_ERR_STATE;
+  }
+  std::vector<double> K[NKC];
+  for (auto &v : K) v.assign((size_t)n, 0.0);
+  // 1/dx as the float the reference's kernels hold (`float H`, kernel.cu:158)
+  KernCtx cx{coef_ref, (double)(1.0f / g.dx), K};
+  parallel_for(n, kern_body, &cx);
+  return upload_kernel_coefs(K);
+}
+
+// Scales (16-bit modes) and uploads kernel coefficient arrays.
 // FP16 storage: each field group is stored divided by a scale (stresses/pressure by the
 // peak source amplitude A; velocities by A*sqrt(max U1 / max S2), which makes the
 // velocity and stress update coefficients both equal to the CFL number c*dt/dx; the
 // relative fluid velocity by its forced response A * max((|W2|+|W3|)/(1-W1)) * 0.2, the
 // 0.2 being k*dx of a wave sampled at ~30 points per wavelength); every coefficient array
 // is stored divided by its largest magnitude so it lies in [-1, 1].
-int biot_handle::upload_coefficients()
+int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
 {
   std::vector<double> K[NKC];
-  for (auto &v : K) v.assign((size_t)n, 0.0);
-  // 1/dx as the float the reference's kernels hold (`float H`, kernel.cu:158)
-  KernCtx cx{coef_ref, (double)(1.0f / g.dx), K};
-  parallel_for(n, kern_body, &cx);
+  for (int k = 0; k < NKC; ++k) K[k] = K_in[k];
 
   float old_scale[NFIELD];
   std::memcpy(old_scale, fscale, sizeof old_scale);

@@ -44,6 +44,7 @@ struct biot_handle {
   // receivers
   void *d_rec = nullptr;
   void *d_traces = nullptr;
+  double *d_traces64 = nullptr;  // FP64 handles: the traces in double as well
   int64_t nrec = 0;
   uint32_t rec_mask = 0;
   int comp[biot::NFIELD] = {};
@@ -62,6 +63,22 @@ struct biot_handle {
   int allocate();
   int upload_profiles();
   int upload_coefficients();
+  int upload_kernel_coefs(const std::vector<double> (&K)[biot::NKC]);
+
+  // inversion parameters (biot_set_params) and the adjoint's buffers
+  std::vector<double> par[8];                // lambda mu alpha M rho rhof C1 tphi
+  std::vector<double> K_last[biot::NKC];     // kernel coefficients built from par
+  std::vector<void *> ckpt;                  // checkpoint states, every ckpt_every steps
+  int ckpt_every = 0, ckpt_nt = 0;
+  void *d_slots = nullptr;                   // segment states (ckpt_every + 1)
+  void *d_lam = nullptr;                     // adjoint fields, carries, derivative adjoints
+  double *d_grad = nullptr;                  // 22 FP64 coefficient gradients
+  float *d_adjsrc = nullptr;                 // adjoint source traces
+  size_t state_bytes() const;
+  void save_state(void *dst);
+  void load_state(const void *src);
+  void free_adjoint();
+  void forget_model();  // drop parameter-built coefficients and every adjoint buffer
   // marks: null, or 3 events per step recorded before velocity, between, after stress
   int enqueue_steps(int nsteps, hipEvent_t *marks);
 };

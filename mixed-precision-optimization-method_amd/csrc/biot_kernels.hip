@@ -477,9 +477,12 @@ __global__ void k_record(const RecordArgs ra)
   const int ci = blockIdx.y;
   if (r >= ra.nrec || ci >= ra.ncomp) return;
   const int f = ra.comp[ci];
-  const float v = (float)load_raw<P>(ra.f[f], ra.lo[f], (COMP_FIELDS >> f) & 1u, (u32)ra.rec[r]) *
-                  ra.scale[ci];
-  ra.traces[((int64_t)ci * ra.nt_max + ra.it) * ra.nrec + r] = v;
+  // the product of two floats is exact in double, so (float)v is the rounded float product
+  const double v = (double)load_raw<P>(ra.f[f], ra.lo[f], (COMP_FIELDS >> f) & 1u, (u32)ra.rec[r]) *
+                   (double)ra.scale[ci];
+  const int64_t o = ((int64_t)ci * ra.nt_max + ra.it) * ra.nrec + r;
+  ra.traces[o] = (float)v;
+  if (ra.traces64) ra.traces64[o] = v;
 }
 
 // In-place multiply of a stored array (re-expresses values when a storage scale changes).

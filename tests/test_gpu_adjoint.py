@@ -1,0 +1,159 @@
+"""GPU checks of the hand-written adjoint (biot_adjoint through the C-ABI).
+
+* Dot-product test: for random trace weights y and a random smooth perturbation dp of all
+  six parameters, <J dp, y> from central differences of FP64 forward runs equals
+  <dp, J^T y> from the adjoint (FP64 adjoint: 1e-7; FP32 adjoint: 1e-4).
+* Finite differences at 8 seeded cells of the 128^3, 1000-step sphere workload: dJ/dp of
+  every parameter against central differences of the FP64 forward misfit (bars in the
+  test's docstring).
+* The torch.autograd binding returns the same gradients as the direct call.
+"""
+import numpy as np
+import pytest
+import torch
+
+from biot_amd import BiotError, autograd, configs, media
+from biot_amd._lib import PARAM_NAMES
+
+pytestmark = pytest.mark.gpu
+
+
+def _start_params(cfg_start):
+    return media.params_from_medium(cfg_start["medium"])
+
+
+def _forward_f64(s, params, nt):
+    s.set_params(params)
+    s.reset()
+    s.run(nt)
+    return s.traces_f64()
+
+
+def _smooth_direction(shape, seed):
+    rng = np.random.default_rng(seed)
+    out = {}
+    for q, k in enumerate(PARAM_NAMES):
+        out[k] = media.gaussian_field(shape, 3.0, seed + 17 * q) * (0.5 + rng.random())
+    return out
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-7), ("fp32", 1e-4)])
+def test_adjoint_dot_product(precision, tol):
+    cfg = configs.adjoint128(n=48, nt=300, every=40, sphere=False)
+    nt, every = cfg["nt"], cfg["every"]
+    print(f"dot product {precision} 48^3 {nt} steps", flush=True)
+    p0 = _start_params(cfg)
+    shape = p0["rho"].shape
+    d = _smooth_direction(shape, 7)
+    s64 = configs.make_solver(cfg, precision="fp64")
+    eps = 1e-5
+    plus = dict(p0, **{k: p0[k] * (1 + eps * d[k]) for k in PARAM_NAMES})
+    minus = dict(p0, **{k: p0[k] * (1 - eps * d[k]) for k in PARAM_NAMES})
+    dd = (_forward_f64(s64, plus, nt) - _forward_f64(s64, minus, nt)) / (2 * eps)
+    y = np.random.default_rng(11).standard_normal(dd.shape).astype(np.float32)
+    lhs = float(np.sum(dd * y.astype(np.float64)))
+    s = s64 if precision == "fp64" else configs.make_solver(cfg, precision=precision)
+    s.set_params(p0)
+    s.forward_checkpointed(nt, every)
+    g, _ = s.adjoint(y)
+    rhs = float(sum(np.sum(g[k] * p0[k] * d[k]) for k in PARAM_NAMES))
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"  <J dp, y> = {lhs:.12e}  <dp, J^T y> = {rhs:.12e}  rel {rel:.2e}")
+    assert rel <= tol
+
+
+def _fd_cells(cfg, count, seed):
+    rng = np.random.default_rng(seed)
+    lo, hi = 24, cfg["nx"] - 24
+    return [tuple(int(v) for v in rng.integers(lo, hi, 3)) for _ in range(count)]
+
+
+def test_adjoint_matches_finite_differences_at_seeded_cells():
+    """128^3, 1000 steps, checkpoints every 50: L2 misfit against traces of the +8 % Kb
+    sphere model, gradients of the start model at 8 seeded cells for all six parameters,
+    against central differences (step 2e-4 of the value) of the FP64 forward misfit.
+
+    Bars (relative, per cell and parameter): the FP64 adjoint 1e-3; the FP32 adjoint fed the
+    FP64 residual 1e-3.  The FP32 run end to end (FP32 forward residual) is bounded by the
+    FP32 forward's trace error relative to the residual, not by the adjoint: the residual is
+    ~2e-3 of the data while FP32 traces carry ~3e-6 relative error, so the residual itself
+    is off by ~2e-3 and single-cell gradients (time integrals with heavy cancellation) by up
+    to a few 1e-2; that bar is 5e-2."""
+    true = configs.adjoint128()
+    start = configs.adjoint128(sphere=False)
+    nt, every = start["nt"], start["every"]
+    print("adjoint128 fd check", flush=True)
+    s64 = configs.make_solver(start, precision="fp64")
+    obs = _forward_f64(s64, media.params_from_medium(true["medium"]), nt).astype(np.float32)
+    p0 = _start_params(start)
+    s64.set_params(p0)
+    s64.forward_checkpointed(nt, every)
+    J0, res = s64.misfit(obs)
+    g64, _ = s64.adjoint(res)
+    s32 = configs.make_solver(start, precision="fp32")
+    s32.set_params(p0)
+    s32.forward_checkpointed(nt, every)
+    _, res32 = s32.misfit(obs)
+    g32r64, _ = s32.adjoint(res)
+    g32, _ = s32.adjoint(res32)
+    print(f"  J0 = {J0:.6e}", flush=True)
+    worst = dict(fp64=0.0, fp32_res64=0.0, fp32=0.0)
+    for (i, j, k) in _fd_cells(start, 8, 2024):
+        for name in PARAM_NAMES:
+            h = 2e-4 * abs(p0[name][k, j, i])
+            Js = []
+            for sgn in (1, -1):
+                p = {kk: v.copy() for kk, v in p0.items()}
+                p[name][k, j, i] += sgn * h
+                _forward_f64(s64, p, nt)
+                Js.append(s64.misfit(obs)[0])
+            fd = (Js[0] - Js[1]) / (2 * h)
+            errs = {key: abs(g[name][k, j, i] - fd) / abs(fd)
+                    for key, g in (("fp64", g64), ("fp32_res64", g32r64), ("fp32", g32))}
+            for key, e in errs.items():
+                worst[key] = max(worst[key], e)
+            print(f"  ({i},{j},{k}) {name:6s} fd {fd:+.6e} adj64 {g64[name][k, j, i]:+.6e} "
+                  + " ".join(f"{key} {e:.1e}" for key, e in errs.items()), flush=True)
+    print("  worst: " + " ".join(f"{key} {e:.2e}" for key, e in worst.items()))
+    assert worst["fp64"] <= 1e-3
+    assert worst["fp32_res64"] <= 1e-3
+    assert worst["fp32"] <= 5e-2
+
+
+def test_autograd_binding_matches_direct_adjoint():
+    cfg = configs.adjoint128(n=40, nt=200, every=30, sphere=False)
+    nt, every = cfg["nt"], cfg["every"]
+    true = configs.adjoint128(n=40, nt=200, every=30, sphere=True)
+    s = configs.make_solver(cfg, precision="fp32")
+    s.set_params(media.params_from_medium(true["medium"]))
+    s.reset()
+    s.run(nt)
+    obs = s.traces()
+    p0 = _start_params(cfg)
+    J, g_direct = autograd.misfit_gradient(p0, s, obs, nt, every)
+    tens = {k: torch.tensor(v, dtype=torch.float64, requires_grad=k in PARAM_NAMES)
+            for k, v in p0.items()}
+    d = autograd.traces(tens, s, nt, every)
+    loss = 0.5 * ((d.double() - torch.from_numpy(obs).double()) ** 2).sum()
+    loss.backward()
+    assert abs(loss.item() - J) <= 1e-6 * J
+    for k in PARAM_NAMES:
+        a, b = tens[k].grad.numpy(), g_direct[k]
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max(), k
+
+
+def test_adjoint_argument_errors():
+    cfg = configs.adjoint128(n=32, nt=50, every=10, sphere=False)
+    s = configs.make_solver(cfg, precision="fp32")
+    with pytest.raises(BiotError):  # no checkpointed run yet
+        s.adjoint(np.zeros((s.ncomp, s.nt_max, s.nrec), np.float32))
+    with pytest.raises(BiotError):  # more steps than the receivers hold
+        s.forward_checkpointed(cfg["nt"] + 1, 10)
+    s16 = configs.make_solver(cfg, precision="fp16")
+    with pytest.raises(BiotError):  # the adjoint path is FP32/FP64
+        s16.forward_checkpointed(cfg["nt"], 10)
+    s.forward_checkpointed(cfg["nt"], 10)
+    with pytest.raises(BiotError):  # parameter gradients need biot_set_params
+        s.adjoint(np.zeros((s.ncomp, s.nt_max, s.nrec), np.float32), params=True)
+    _, gk = s.adjoint(np.zeros((s.ncomp, s.nt_max, s.nrec), np.float32), params=False, coefs=True)
+    assert all(not np.any(x) for x in gk)
