@@ -142,6 +142,45 @@ def cpu_baseline(cfg, nsteps):
                        f"(-O3, OpenMP), {dt:.1f} s")
 
 
+def adjoint_shot(repeats=2):
+    """Adjoint gradient of the 128^3, 1000-step sphere workload (checkpoints every 50, FP32
+    wavefields, FP64 gradient sums): forward with checkpoints, FP64 misfit, segment
+    recompute and adjoint sweep, per shot.  Observed traces come from the true model."""
+    import numpy as np
+
+    from biot_amd import configs, media
+
+    true, start = configs.adjoint128(), configs.adjoint128(sphere=False)
+    nt, every = start["nt"], start["every"]
+    s = configs.make_solver(start, precision="fp32")
+    s.set_params(media.params_from_medium(true["medium"]))
+    s.reset()
+    s.run(nt)
+    obs = s.traces()
+    s.set_params(media.params_from_medium(start["medium"]))
+    best = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        s.forward_checkpointed(nt, every)
+        f_ms = s.last_run_ms()
+        J, res = s.misfit(obs)
+        g, _ = s.adjoint(res)
+        a_ms = s.last_run_ms()
+        wall = (time.perf_counter() - t0) * 1e3
+        r = dict(ms_per_shot=round(f_ms + a_ms, 2), forward_checkpointed_ms=round(f_ms, 2),
+                 adjoint_ms=round(a_ms, 2), wall_ms_per_shot=round(wall, 1), misfit=J,
+                 finite=bool(all(np.isfinite(v).all() for v in g.values())))
+        if best is None or r["ms_per_shot"] < best["ms_per_shot"]:
+            best = r
+    s.close()
+    best.update(workload="adjoint128: 128^3, order 4, PML 10, 1000 steps, checkpoint every 50, "
+                         "FP32 wavefields, FP64 gradient sums, 512 receivers x (vx, vz, p)",
+                unit="ms/shot", note="ms_per_shot: HIP-event time of the checkpointed forward "
+                "plus recompute, adjoint sweep and FP64 gradient sums; wall adds misfit, "
+                "gradient download and the host chain rule to the six parameters")
+    return best
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB (torch's copy kernel), repeated 5 times:
     the HBM-traffic counters of this dispatch calibrate FETCH_SIZE / WRITE_SIZE."""
@@ -166,6 +205,7 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.copy_calib:
         copy_calibration(args.copy_calib)
@@ -236,6 +276,10 @@ def main():
                      "kernel_ms_v": round(r16["kernel_ms_v"], 5),
                      "kernel_ms_t": round(r16["kernel_ms_t"], 5), "finite": r16["finite"],
                      "tile_bx_by_zchunk": r16["shape"]}
+    if not args.no_adjoint:
+        log("[bench] adjoint 128^3 x 1000 steps ...")
+        out["adjoint"] = adjoint_shot()
+        log(f"[bench] adjoint: {out['adjoint']['ms_per_shot']:.0f} ms/shot")
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)

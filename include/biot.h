@@ -161,7 +161,7 @@ typedef struct biot_params {
 } biot_params;
 int biot_set_params(biot_handle *h, const biot_params *p);
 /* Resets, runs nt steps recording traces (receivers must hold >= nt steps) and keeps the
- * state every `every` steps on the device for the adjoint.  FP32 and FP64 handles. */
+ * state every `every` steps on the device for the adjoint.  FP32, FP64 and FP16 handles. */
 int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every);
 /* J = 1/2 sum (d - observed)^2 over the recorded traces (the double traces on FP64
  * handles), reduced in FP64 on the device; residual (may be NULL) receives d - observed in
@@ -189,11 +189,64 @@ int biot_params_coefficients(const biot_grid *g, const biot_params *p, double *c
 int biot_params_chain(const biot_grid *g, const biot_params *p, const double *const grad_coef[22],
                       double *const grad_params[6]);
 
+/* ---- batched shots (no counterpart in the reference) ---- */
+/* Many shots, each a window of one model grid with its own absorbing boundary, source and
+ * receivers, advanced together: one persistent launch per half step covers the tiles of
+ * every shot, and runs of steps are captured into a HIP graph and replayed.  The gradient
+ * sums every shot's parameter gradient into one model-grid volume on the device.
+ * Precisions FP32, FP16 (binary16 wavefields, float arithmetic, FP64 gradient sums), FP64. */
+typedef struct biot_batch biot_batch;
+typedef struct biot_batch_config {
+  int32_t nx, ny, nz;        /* model grid the windows are cut from (gradient volume) */
+  int32_t order, precision, npml;
+  float dx, dt, pml_v0, pml_R, Cv, Cp;
+  int32_t device;
+} biot_batch_config;
+typedef struct biot_shot {
+  int32_t x0, y0, z0;        /* window origin in the model grid */
+  int32_t nx, ny, nz;        /* window size (x0 + nx <= model nx, ...) */
+  int64_t src_cell;          /* window-local linear cell */
+  uint32_t src_mask;         /* stresses / pressure only for gradients */
+  const int64_t *rec_cells;  /* window-local linear cells */
+  int64_t nrec;
+  uint32_t rec_mask;
+} biot_shot;
+int biot_batch_create(const biot_batch_config *c, biot_batch **out);
+int biot_batch_destroy(biot_batch *b);
+const char *biot_batch_last_error(const biot_batch *b);
+/* window_params: the window's cells (its nx*ny*nz); receivers hold nt steps */
+int biot_batch_add_shot(biot_batch *b, const biot_shot *s, const biot_params *window_params,
+                        const float *wavelet, int32_t nt);
+int biot_batch_shot_count(biot_batch *b, int32_t *n);
+/* the per-shot grid (for reading fields or traces through the single-grid calls) */
+int biot_batch_shot_handle(biot_batch *b, int32_t shot, biot_handle **h);
+/* tile shape and persistent grid size (blocks, a multiple of 8) of the batched launches */
+int biot_batch_set_launch(biot_batch *b, int32_t bx, int32_t by, int32_t zchunk, int32_t blocks);
+int biot_batch_get_launch(biot_batch *b, int32_t *bx, int32_t *by, int32_t *zchunk, int32_t *blocks);
+/* times tile shapes and grid sizes over trial_steps steps and keeps the fastest; leaves
+ * every shot reset */
+int biot_batch_autotune(biot_batch *b, int32_t trial_steps);
+/* every shot from rest for nt steps, recording traces; graph_steps > 0 captures that many
+ * steps into a HIP graph and replays it (the remainder runs step by step) */
+int biot_batch_forward(biot_batch *b, int32_t nt, int32_t graph_steps);
+int biot_batch_read_traces(biot_batch *b, int32_t shot, float *out, int64_t count);
+/* J = sum over shots of 1/2 |d - observed|^2 (FP64 on the device) and its gradient with
+ * respect to (lambda, mu, alpha, M, rho, rho_f) on the model grid (6 arrays of nx*ny*nz),
+ * through checkpoints every `every` steps and the adjoint of every shot at once */
+int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, const float *const observed[],
+                        double *J, double *const grad[6]);
+/* HIP-event time of the last biot_batch_forward or biot_batch_gradient */
+int biot_batch_last_ms(biot_batch *b, double *ms);
+/* frees the gradient's checkpoints and adjoint buffers */
+int biot_batch_release(biot_batch *b);
+
 /* ---- launch shapes ---- */
 /* Tile shape of the two stencil kernels: bx*by threads per block (bx*by <= 256, bx a
- * multiple of 64), each thread marching zchunk planes. */
-int biot_set_launch_shape(biot_handle *h, int32_t bx, int32_t by, int32_t zchunk);
-int biot_get_launch_shape(biot_handle *h, int32_t *bx, int32_t *by, int32_t *zchunk);
+ * multiple of 64), each thread marching zchunk planes and updating xv consecutive x cells
+ * (1; or 2 and 4 for FP32/FP16, 4 for FP16C, when nx is a multiple of 4).  xv may be NULL
+ * in biot_get_launch_shape. */
+int biot_set_launch_shape(biot_handle *h, int32_t bx, int32_t by, int32_t zchunk, int32_t xv);
+int biot_get_launch_shape(biot_handle *h, int32_t *bx, int32_t *by, int32_t *zchunk, int32_t *xv);
 /* Times every candidate tile shape for trial_steps steps on this grid and keeps the
  * fastest; results are cached per (precision, order, grid, GPU arch) in the file named by
  * $BIOT_TUNE_CACHE (default $HOME/.cache/biot_tune.txt), and a cached entry is reused

@@ -67,6 +67,22 @@ NKCOEF = 22
 KCOEF_NAMES = ["U1X", "U1Y", "U1Z", "U2X", "U2Y", "U2Z", "W1X", "W1Y", "W1Z", "W2X", "W2Y",
                "W2Z", "W3X", "W3Y", "P1", "P2", "S1", "S2", "S3", "SXY", "SXZ", "SYZ"]
 
+class BatchConfig(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("order", C.c_int32), ("precision", C.c_int32), ("npml", C.c_int32),
+                ("dx", C.c_float), ("dt", C.c_float), ("pml_v0", C.c_float),
+                ("pml_R", C.c_float), ("Cv", C.c_float), ("Cp", C.c_float),
+                ("device", C.c_int32)]
+
+
+class Shot(C.Structure):
+    _fields_ = [("x0", C.c_int32), ("y0", C.c_int32), ("z0", C.c_int32),
+                ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("src_cell", C.c_int64), ("src_mask", C.c_uint32),
+                ("rec_cells", C.POINTER(C.c_int64)), ("nrec", C.c_int64),
+                ("rec_mask", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -105,8 +121,9 @@ def lib():
         "biot_synchronize": ([H], C.c_int),
         "biot_last_run_ms": ([H, _DP], C.c_int),
         "biot_profile_steps": ([H, i32, _DP, _DP], C.c_int),
-        "biot_set_launch_shape": ([H, i32, i32, i32], C.c_int),
-        "biot_get_launch_shape": ([H, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
+        "biot_set_launch_shape": ([H, i32, i32, i32, i32], C.c_int),
+        "biot_get_launch_shape": ([H, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                   C.POINTER(i32)], C.c_int),
         "biot_autotune": ([H, i32], C.c_int),
         "biot_read_traces": ([H, FP, i64], C.c_int),
         "biot_read_traces_f64": ([H, _DP, i64], C.c_int),
@@ -123,6 +140,21 @@ def lib():
         "biot_params_coefficients": ([C.POINTER(Grid), C.POINTER(Params), C.POINTER(_DP)], C.c_int),
         "biot_params_chain": ([C.POINTER(Grid), C.POINTER(Params), C.POINTER(_DP), C.POINTER(_DP)],
                               C.c_int),
+        "biot_batch_create": ([C.POINTER(BatchConfig), C.POINTER(H)], C.c_int),
+        "biot_batch_destroy": ([H], C.c_int),
+        "biot_batch_last_error": ([H], C.c_char_p),
+        "biot_batch_add_shot": ([H, C.POINTER(Shot), C.POINTER(Params), FP, i32], C.c_int),
+        "biot_batch_shot_count": ([H, C.POINTER(i32)], C.c_int),
+        "biot_batch_shot_handle": ([H, i32, C.POINTER(H)], C.c_int),
+        "biot_batch_set_launch": ([H, i32, i32, i32, i32], C.c_int),
+        "biot_batch_get_launch": ([H, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                   C.POINTER(i32)], C.c_int),
+        "biot_batch_autotune": ([H, i32], C.c_int),
+        "biot_batch_forward": ([H, i32, i32], C.c_int),
+        "biot_batch_read_traces": ([H, i32, FP, i64], C.c_int),
+        "biot_batch_gradient": ([H, i32, i32, C.POINTER(FP), _DP, C.POINTER(_DP)], C.c_int),
+        "biot_batch_last_ms": ([H, _DP], C.c_int),
+        "biot_batch_release": ([H], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -145,7 +177,12 @@ def exported_symbols():
             "biot_read_field_f64", "biot_write_field_f64", "biot_snapshot_xz",
             "biot_get_field_scales", "biot_device_bytes", "biot_set_params",
             "biot_forward_checkpointed", "biot_misfit_l2", "biot_adjoint",
-            "biot_params_coefficients", "biot_params_chain"]
+            "biot_params_coefficients", "biot_params_chain", "biot_batch_create",
+            "biot_batch_destroy", "biot_batch_last_error", "biot_batch_add_shot",
+            "biot_batch_shot_count", "biot_batch_shot_handle", "biot_batch_set_launch",
+            "biot_batch_get_launch", "biot_batch_autotune", "biot_batch_forward",
+            "biot_batch_read_traces", "biot_batch_gradient", "biot_batch_last_ms",
+            "biot_batch_release"]
 
 
 def fptr(a):

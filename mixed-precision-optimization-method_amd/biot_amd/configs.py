@@ -73,6 +73,56 @@ def adjoint128(order=4, nt=1000, n=128, every=50, sphere=True):
                 medium=m, rec=rec, rec_fields=REC_FIELDS)
 
 
+def batch16(nshots=16, nt=1000, model=(512, 512, 384), side=(96, 192), seed=7, npml=20,
+            order=4, rec_stride=8, medium=None):
+    """Uneven cubic windows (side ~ U[side], seeded) at seeded positions inside the model
+    grid (nx, ny, nz); the medium is the 512 x 512 x 384 perturbed one unless given (arrays
+    (nz, ny, nx)).  Each shot: 20 Hz explosive source at the window centre, receivers every
+    rec_stride cells on the window plane z = npml recording vx, vz, p."""
+    nx, ny, nz = model
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(side[0], side[1] + 1, nshots)
+    if medium is None:
+        medium = media.perturbed((nz, ny, nx), seed=2024)
+    shots = []
+    for s in sides:
+        s = int(s)
+        x0 = int(rng.integers(0, nx - s + 1))
+        y0 = int(rng.integers(0, ny - s + 1))
+        z0 = int(rng.integers(0, nz - s + 1))
+        g = np.arange(npml, s - npml, rec_stride)
+        ii, jj = np.meshgrid(g, g, indexing="xy")
+        rec = np.stack([ii.ravel(), jj.ravel(), np.full(ii.size, npml)], axis=1)
+        shots.append(dict(origin=(x0, y0, z0), size=(s, s, s), src=(s // 2, s // 2, s // 2), rec=rec))
+    return dict(name="batch16", model=model, medium=medium, shots=shots, dx=DX, dt=DT, order=order,
+                npml=npml, nt=nt, f0=20.0, src_fields=SRC_EXPLOSIVE, rec_fields=REC_FIELDS)
+
+
+def window_params(cfg, shot):
+    """The inversion parameters of one shot's window, (nz, ny, nx) arrays."""
+    x0, y0, z0 = shot["origin"]
+    wx, wy, wz = shot["size"]
+    sub = media.crop(cfg["medium"], z0, y0, x0, (wz, wy, wx))
+    return media.params_from_medium(sub)
+
+
+def make_batch(cfg, precision="fp16", params=None):
+    """A Batch with every shot of a batch workload added (params: per-shot parameter dicts,
+    default the workload's medium)."""
+    from ._lib import ricker
+    from .batch import Batch
+
+    nx, ny, nz = cfg["model"]
+    b = Batch(nx, ny, nz, cfg["dx"], cfg["dt"], order=cfg["order"], precision=precision,
+              npml=cfg["npml"])
+    w = ricker(cfg["f0"], cfg["dt"], cfg["nt"])
+    for i, sh in enumerate(cfg["shots"]):
+        p = window_params(cfg, sh) if params is None else params[i]
+        b.add_shot(sh["origin"], sh["size"], p, sh["src"], w, sh["rec"], cfg["rec_fields"],
+                   cfg["src_fields"])
+    return b
+
+
 def cells(cfg, ijk):
     ijk = np.asarray(ijk, dtype=np.int64).reshape(-1, 3)
     return ijk[:, 0] + ijk[:, 1] * cfg["nx"] + ijk[:, 2] * cfg["nx"] * cfg["ny"]

@@ -1,4 +1,5 @@
-// Discrete adjoint of the Biot time step (HIP, gfx950), FP32 or FP64 storage.
+// Discrete adjoint of the Biot time step (HIP, gfx950): FP32, FP64 or FP16 forward states,
+// adjoint wavefields in the arithmetic type (float, double for FP64), FP64 gradient sums.
 //
 // One forward step n maps u^n (after the source of step n) to u^{n+1}:
 //   velocity half step  D = delta(tau, p); D' = (1+a) D + c_V; c_V <- e (c_V + 2 a D)
@@ -14,6 +15,8 @@
 //            forward derivatives recomputed from the stored states;
 //   gather : lambda_f(c) += sum_m w_m G(c - o_m), the transposed staggered differences,
 //            written per cell so the result is deterministic (no atomics).
+// With 16-bit storage everything runs in storage units (fields divided by their scales,
+// coefficients as the kernels apply them); the host converts dJ/dK with the per-array factor.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -50,9 +53,11 @@ __constant__ Der TDER[12] = {{0, 0, 0, 4, 0}, {1, 1, 0, 4, 0}, {2, 2, 0, 4, 0}, 
                              {4, 1, 0, 5, 0}, {5, 2, 0, 5, 0}, {0, 1, 1, 6, 1}, {1, 0, 1, 6, 1},
                              {0, 2, 1, 6, 1}, {2, 0, 1, 7, 1}, {1, 2, 1, 7, 1}, {2, 1, 1, 7, 1}};
 
+// storage type S -> arithmetic type A (also the C-PML carry type)
 template <typename S>
 struct AdjT {
   using A = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
+  static constexpr bool scaled = std::is_same<S, _Float16>::value;
 };
 
 struct CellIdx {
@@ -61,7 +66,8 @@ struct CellIdx {
 };
 
 // update-box cell t (x fastest)
-__device__ __forceinline__ bool cell_of(const AdjArgs &a, int h, u32 t, CellIdx &ci)
+template <class AT>
+__device__ __forceinline__ bool cell_of(const AT &a, int h, u32 t, CellIdx &ci)
 {
   const u32 ni = (u32)(a.nx - 2 * h), nj = (u32)(a.ny - 2 * h);
   ci.i = h + (int)(t % ni);
@@ -70,6 +76,17 @@ __device__ __forceinline__ bool cell_of(const AdjArgs &a, int h, u32 t, CellIdx 
   ci.k = h + (int)(r / nj);
   ci.c = (u32)ci.i + (u32)ci.j * (u32)a.sy + (u32)ci.k * (u32)a.sz;
   return ci.k < a.nz - h;
+}
+
+template <class AT>
+__device__ __forceinline__ const void *fw_field(const AT &a, int q, int f)
+{
+  return a.slots + (uint64_t)q * a.sb + (uint64_t)f * a.fbytes;
+}
+template <class AT>
+__device__ __forceinline__ const void *fw_carry(const AT &a, int q, int ax)
+{
+  return a.slots + (uint64_t)q * a.sb + a.coff[ax];
 }
 
 template <typename S, int ORDER>
@@ -108,11 +125,12 @@ __device__ __forceinline__ int slab_pos(int i, int n, int Lw, int Rw)
 }
 
 // carry element index of slot `slot` on axis `ax` at cell ci, or -1 outside that slab
-__device__ __forceinline__ long long carry_idx(const AdjArgs &a, int ax, int slot, const CellIdx &ci)
+template <class AT>
+__device__ __forceinline__ long long carry_idx(const AT &a, int ax, int slot, const CellIdx &ci)
 {
-  const int pos[3] = {ci.i, ci.j, ci.k};
-  const int nn[3] = {a.nx, a.ny, a.nz};
-  const int p = slab_pos(pos[ax], nn[ax], a.L[ax], a.R[ax]);
+  const int pos = ax == 0 ? ci.i : (ax == 1 ? ci.j : ci.k);
+  const int nn = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
+  const int p = slab_pos(pos, nn, a.L[ax], a.R[ax]);
   if (p < 0) return -1;
   const u32 w = (u32)(a.L[ax] + a.R[ax]);
   u32 o;
@@ -122,30 +140,32 @@ __device__ __forceinline__ long long carry_idx(const AdjArgs &a, int ax, int slo
   return (long long)((u32)slot * (u32)a.slab[ax] + o);
 }
 
-template <typename A>
-__device__ __forceinline__ A profv(const AdjArgs &a, int ax, int q, const CellIdx &ci)
+template <typename A, class AT>
+__device__ __forceinline__ A profv(const AT &a, int ax, int q, const CellIdx &ci)
 {
-  const int pos[3] = {ci.i, ci.j, ci.k};
-  return at<A>(a.prof[ax][q], (u32)pos[ax]);
+  const int pos = ax == 0 ? ci.i : (ax == 1 ? ci.j : ci.k);
+  return at<A>(a.prof[ax][q], (u32)pos);
 }
 
-template <typename S>
-__device__ __forceinline__ typename AdjT<S>::A kcoef(const AdjArgs &a, int kidx, u32 c)
-{
-  return (typename AdjT<S>::A)at<S>(a.k[kidx], c);
-}
-
-// post-PML forward derivative d (table entry D) and the carry adjoint update
-template <typename S, int ORDER>
-__device__ __forceinline__ void pml_adjoint(const AdjArgs &a, const Der &D, const CellIdx &ci,
-                                            const void *const *fwd_fields, const void *const *fwd_carry,
-                                            typename AdjT<S>::A lamEp, typename AdjT<S>::A &fwdEp,
-                                            typename AdjT<S>::A &G)
+template <typename S, class AT>
+__device__ __forceinline__ typename AdjT<S>::A kcoef(const AT &a, int kidx, u32 c)
 {
   using A = typename AdjT<S>::A;
-  using SC = S;
+  const A v = (A)at<S>(a.k[kidx], c);
+  if constexpr (AdjT<S>::scaled) return v * (A)a.ks[kidx];
+  return v;
+}
+
+// post-PML forward derivative (table entry D, fields of state qf, carries of state qc) and
+// the carry adjoint update
+template <typename S, int ORDER, class AT>
+__device__ __forceinline__ void pml_adjoint(const AT &a, const Der &D, const CellIdx &ci, int qf,
+                                            int qc, typename AdjT<S>::A lamEp,
+                                            typename AdjT<S>::A &fwdEp, typename AdjT<S>::A &G)
+{
+  using A = typename AdjT<S>::A;
   const u32 st[3] = {1u, (u32)a.sy, (u32)a.sz};
-  const A E = diff<S, ORDER>(fwd_fields[D.field], ci.c, st[D.dir], D.fwd);
+  const A E = diff<S, ORDER>(fw_field(a, qf, D.field), ci.c, st[D.dir], D.fwd);
   const long long ix = carry_idx(a, D.dir, D.slot, ci);
   if (ix < 0) {
     fwdEp = E;
@@ -154,20 +174,19 @@ __device__ __forceinline__ void pml_adjoint(const AdjArgs &a, const Der &D, cons
   }
   const A e = profv<A>(a, D.dir, D.half ? 2 : 0, ci);
   const A aa = profv<A>(a, D.dir, D.half ? 3 : 1, ci);
-  const A cf = (A)at<SC>(fwd_carry[D.dir], (u32)ix);
+  const A cf = at<A>(fw_carry(a, qc, D.dir), (u32)ix);
   fwdEp = (A(1) + aa) * E + cf;
   A &lc = at<A>(a.lcarry[D.dir], (u32)ix);
   G = (A(1) + aa) * lamEp + A(2) * aa * e * lc;
   lc = lamEp + e * lc;
 }
 
-template <typename S, int ORDER>
-__global__ void __launch_bounds__(256) k_adj_stress_local(const AdjArgs a)
+template <typename S, int ORDER, class AT>
+__device__ __forceinline__ void adj_stress_local(const AT &a, int q, u32 t)
 {
   using A = typename AdjT<S>::A;
   constexpr int h = ORDER / 2;
   CellIdx ci;
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
   const u32 c = ci.c;
   const A ltxx = at<A>(a.lam[6], c), ltyy = at<A>(a.lam[7], c), ltzz = at<A>(a.lam[8], c);
@@ -188,7 +207,8 @@ __global__ void __launch_bounds__(256) k_adj_stress_local(const AdjArgs a)
 #pragma unroll
   for (int d = 0; d < 12; ++d) {
     A G;
-    pml_adjoint<S, ORDER>(a, TDER[d], ci, a.fw1, a.fcarry, lE[d], E[d], G);
+    // velocities of state q+1 (after the velocity half step), carries of state q
+    pml_adjoint<S, ORDER>(a, TDER[d], ci, q + 1, q, lE[d], E[d], G);
     at<A>(a.G[d], c) = G;
   }
   // FP64 gradient sums: dJ/dK += lambda(output) x (forward term), kernel.cu:484-490
@@ -204,13 +224,12 @@ __global__ void __launch_bounds__(256) k_adj_stress_local(const AdjArgs a)
 }
 
 // lambda(v', q') += transposed differences of the stress-side derivative adjoints
-template <typename S, int ORDER>
-__global__ void __launch_bounds__(256) k_adj_stress_gather(const AdjArgs a)
+template <typename S, int ORDER, class AT>
+__device__ __forceinline__ void adj_stress_gather(const AT &a, int, u32 t)
 {
   using A = typename AdjT<S>::A;
   constexpr int h = ORDER / 2;
   CellIdx ci;
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
   const u32 c = ci.c, sx = 1, sy = (u32)a.sy, sz = (u32)a.sz;
   at<A>(a.lam[0], c) += diffT<A, ORDER>(a.G[0], c, sx, 0) + diffT<A, ORDER>(a.G[6], c, sy, 1) +
@@ -224,13 +243,12 @@ __global__ void __launch_bounds__(256) k_adj_stress_gather(const AdjArgs a)
   at<A>(a.lam[5], c) += diffT<A, ORDER>(a.G[5], c, sz, 0);
 }
 
-template <typename S, int ORDER>
-__global__ void __launch_bounds__(256) k_adj_velocity_local(const AdjArgs a)
+template <typename S, int ORDER, class AT>
+__device__ __forceinline__ void adj_velocity_local(const AT &a, int q, u32 t)
 {
   using A = typename AdjT<S>::A;
   constexpr int h = ORDER / 2;
   CellIdx ci;
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
   const u32 c = ci.c;
   // per component: lambda_qe = lambda_q' - U2 lambda_v'; lambda_S = U1 lambda_v' - W2 lambda_qe;
@@ -240,15 +258,15 @@ __global__ void __launch_bounds__(256) k_adj_velocity_local(const AdjArgs a)
   const int kW1[3] = {K_W1X, K_W1Y, K_W1Z}, kW2[3] = {K_W2X, K_W2Y, K_W2Z};
   const int kW3[3] = {K_W3X, K_W3Y, K_W3Y};
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    lv[q] = at<A>(a.lam[q], c);
-    const A lq = at<A>(a.lam[3 + q], c);
-    const A U1 = kcoef<S>(a, kU1[q], c), U2 = kcoef<S>(a, kU2[q], c);
-    const A W1 = kcoef<S>(a, kW1[q], c), W2 = kcoef<S>(a, kW2[q], c), W3 = kcoef<S>(a, kW3[q], c);
-    lqe[q] = lq - U2 * lv[q];
-    lS[q] = U1 * lv[q] - W2 * lqe[q];
-    lDs[q] = -W3 * lqe[q];
-    at<A>(a.lam[3 + q], c) = U2 * lv[q] + W1 * lqe[q];  // lambda_q at step n (lambda_v is unchanged)
+  for (int m = 0; m < 3; ++m) {
+    lv[m] = at<A>(a.lam[m], c);
+    const A lq = at<A>(a.lam[3 + m], c);
+    const A U1 = kcoef<S>(a, kU1[m], c), U2 = kcoef<S>(a, kU2[m], c);
+    const A W1 = kcoef<S>(a, kW1[m], c), W2 = kcoef<S>(a, kW2[m], c), W3 = kcoef<S>(a, kW3[m], c);
+    lqe[m] = lq - U2 * lv[m];
+    lS[m] = U1 * lv[m] - W2 * lqe[m];
+    lDs[m] = -W3 * lqe[m];
+    at<A>(a.lam[3 + m], c) = U2 * lv[m] + W1 * lqe[m];  // lambda_q at step n (lambda_v is unchanged)
   }
   // derivative order x1 x2 x3 s1 y1 y2 y3 s2 z1 z2 z3 s3
   const A lD[12] = {lS[0], lS[0], lS[0], lDs[0], lS[1], lS[1], lS[1], lDs[1],
@@ -257,7 +275,7 @@ __global__ void __launch_bounds__(256) k_adj_velocity_local(const AdjArgs a)
 #pragma unroll
   for (int d = 0; d < 12; ++d) {
     A G;
-    pml_adjoint<S, ORDER>(a, VDER[d], ci, a.fw, a.fcarry, lD[d], D[d], G);
+    pml_adjoint<S, ORDER>(a, VDER[d], ci, q, q, lD[d], D[d], G);
     at<A>(a.G[d], c) = G;
   }
   // FP64 gradient sums with the forward terms of kernel.cu:218-225
@@ -265,24 +283,24 @@ __global__ void __launch_bounds__(256) k_adj_velocity_local(const AdjArgs a)
                           (double)D[9] + D[10] + D[8]};
   const double Ds[3] = {(double)D[3], (double)D[7], (double)D[11]};
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const double q0 = (double)at<S>(a.fw[3 + q], c), q1 = (double)at<S>(a.fw1[3 + q], c);
-    a.grad[kU1[q]][c] += (double)lv[q] * Ssum[q];
-    a.grad[kU2[q]][c] += -(double)lv[q] * (q1 - q0);
-    a.grad[kW1[q]][c] += (double)lqe[q] * q0;
-    a.grad[kW2[q]][c] += -(double)lqe[q] * Ssum[q];
-    a.grad[kW3[q]][c] += -(double)lqe[q] * Ds[q];
+  for (int m = 0; m < 3; ++m) {
+    const double q0 = (double)at<S>(fw_field(a, q, 3 + m), c);
+    const double q1 = (double)at<S>(fw_field(a, q + 1, 3 + m), c);
+    a.grad[kU1[m]][c] += (double)lv[m] * Ssum[m];
+    a.grad[kU2[m]][c] += -(double)lv[m] * (q1 - q0);
+    a.grad[kW1[m]][c] += (double)lqe[m] * q0;
+    a.grad[kW2[m]][c] += -(double)lqe[m] * Ssum[m];
+    a.grad[kW3[m]][c] += -(double)lqe[m] * Ds[m];
   }
 }
 
 // lambda(tau, p) += transposed differences of the velocity-side derivative adjoints
-template <typename S, int ORDER>
-__global__ void __launch_bounds__(256) k_adj_velocity_gather(const AdjArgs a)
+template <typename S, int ORDER, class AT>
+__device__ __forceinline__ void adj_velocity_gather(const AT &a, int, u32 t)
 {
   using A = typename AdjT<S>::A;
   constexpr int h = ORDER / 2;
   CellIdx ci;
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
   const u32 c = ci.c, sx = 1, sy = (u32)a.sy, sz = (u32)a.sz;
   at<A>(a.lam[6], c) += diffT<A, ORDER>(a.G[0], c, sx, 1);
@@ -295,15 +313,67 @@ __global__ void __launch_bounds__(256) k_adj_velocity_gather(const AdjArgs a)
                          diffT<A, ORDER>(a.G[11], c, sz, 1);
 }
 
-// lambda(recorded field at receiver) += adjoint source sample of step it
-template <typename A>
-__global__ void k_adj_inject(const InjectArgs a)
+template <typename S, int ORDER, int WHICH, class AT>
+__device__ __forceinline__ void adj_pass(const AT &a, int q, u32 t)
 {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int ci = blockIdx.y;
+  if constexpr (WHICH == 0) adj_stress_local<S, ORDER>(a, q, t);
+  else if constexpr (WHICH == 1) adj_stress_gather<S, ORDER>(a, q, t);
+  else if constexpr (WHICH == 2) adj_velocity_local<S, ORDER>(a, q, t);
+  else adj_velocity_gather<S, ORDER>(a, q, t);
+}
+
+template <typename S, int ORDER, int WHICH>
+__global__ void __launch_bounds__(256) k_adj(const AdjArgs a, int q)
+{
+  adj_pass<S, ORDER, WHICH>(a, q, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// batched: argument tables read through the constant address space (uniform per block)
+typedef __attribute__((address_space(4))) const AdjArgs CAdjArgs;
+typedef __attribute__((address_space(4))) const int CInt;
+typedef __attribute__((address_space(4))) const InjectArgs CInjectArgs;
+typedef __attribute__((address_space(4))) const CopyEntry CCopyEntry;
+
+__device__ __forceinline__ int shot_of_block(const AdjBatch &b, int blk)
+{
+  CInt *bs = (CInt *)b.block_start;
+  int lo = 0, hi = b.nshots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bs[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename S, int ORDER, int WHICH>
+__global__ void __launch_bounds__(256) k_adj_batch(const AdjBatch b, int q)
+{
+  const int blk = blockIdx.x;
+  const int s = shot_of_block(b, blk);
+  const CAdjArgs &a = ((CAdjArgs *)b.shots)[s];
+  const u32 t = (u32)(blk - ((CInt *)b.block_start)[s]) * 256u + threadIdx.x;
+  adj_pass<S, ORDER, WHICH>(a, q, t);
+}
+
+// lambda(recorded field at receiver) += scale x adjoint source sample of step it
+template <typename A, class IT>
+__device__ __forceinline__ void inject_one(const IT &a, int64_t r, int ci, int it)
+{
   if (r >= a.nrec || ci >= a.ncomp) return;
-  const float v = a.src[((int64_t)ci * a.nt + a.it) * a.nrec + r] * a.scale[ci];
+  const float v = a.src[((int64_t)ci * a.nt + it) * a.nrec + r] * a.scale[ci];
   atomicAdd(&((A *)a.lam[a.comp[ci]])[a.rec[r]], (A)v);
+}
+template <typename A>
+__global__ void k_adj_inject(const InjectArgs a, int it)
+{
+  inject_one<A>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.y, it);
+}
+template <typename A>
+__global__ void k_adj_inject_batch(const InjectArgs *tab, int it)
+{
+  const CInjectArgs &a = ((CInjectArgs *)tab)[blockIdx.z];
+  inject_one<A>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.y, it);
 }
 
 // J = 1/2 sum (d - obs)^2 in FP64, residual = d - obs
@@ -327,41 +397,210 @@ __global__ void __launch_bounds__(256) k_misfit(const T *d, const float *obs, fl
   if (threadIdx.x == 0) atomicAdd(J, part[0]);
 }
 
+// ---- chain rule (the device form of the host chain in biot_params.cpp) ----
+__device__ __forceinline__ double harmonic4(double a, double b, double c, double d)
+{
+  if (a == 0 || b == 0 || c == 0 || d == 0) return 0.0;
+  return 1.0 / (0.25 * (1 / a + 1 / b + 1 / c + 1 / d));
+}
+
+__global__ void __launch_bounds__(256) k_chain(const ChainArgs c)
+{
+  const int64_t wsy = c.wx, wsz = (int64_t)c.wx * c.wy;
+  const int64_t nw = wsz * c.wz;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nw) return;
+  const int i = (int)(t % c.wx);
+  const int j = (int)((t / c.wx) % c.wy);
+  const int k = (int)(t / wsz);
+  if (i < 1 || i >= c.wx - 1 || j < 1 || j >= c.wy - 1 || k < 1 || k >= c.wz - 1) return;
+  // p: window cell (parameters, coefficient gradients); g: model cell (output)
+  const int64_t p = t;
+  const int64_t g = (int64_t)(c.x0 + i) + (int64_t)(c.y0 + j) * c.msy + (int64_t)(c.z0 + k) * c.msz;
+  auto gK = [&](int kk) { return c.grad[kk][t] * c.kfac[kk]; };
+  const double *mu = c.par[1], *alp = c.par[2], *M = c.par[3];
+  const double *rho = c.par[4], *rhof = c.par[5], *C1 = c.par[6], *tphi = c.par[7];
+  const double k1 = c.Cv * c.dt / c.Cp;   // velocity coefficient scale
+  const double kk = c.dt * c.Cp / c.Cv;   // stress coefficient scale
+  // stress side: K_P1 = C kk/dx, K_P2 = M kk/dx, K_S1 = lambda_c .., K_S2 = H .., K_S3 = C ..
+  const double al = alp[p], Mp = M[p];
+  const double gC = (gK(K_P1) + gK(K_S3)) * kk * c.Hinv;
+  const double glc = gK(K_S1) * kk * c.Hinv;
+  const double gH = gK(K_S2) * kk * c.Hinv;
+  const double gMd = gK(K_P2) * kk * c.Hinv;
+  atomicAdd(&c.out[0][g], glc + gH);
+  double gmu_c = 2 * gH;
+  atomicAdd(&c.out[2][g], gC * Mp + (glc + gH) * 2 * al * Mp);
+  atomicAdd(&c.out[3][g], gMd + gC * al + (glc + gH) * al * al);
+  // shear: 4-cell harmonic means, d mu_bar / d mu_i = mu_bar^2 / (4 mu_i^2)
+  const int64_t wq[3][4] = {{0, 1, wsy, wsy + 1}, {0, 1, wsz, wsz + 1}, {0, wsy, wsz, wsz + wsy}};
+  const int64_t mq[3][4] = {{0, 1, c.msy, c.msy + 1}, {0, 1, c.msz, c.msz + 1},
+                            {0, c.msy, c.msz, c.msz + c.msy}};
+  const int kidx[3] = {K_SXY, K_SXZ, K_SYZ};
+  for (int m = 0; m < 3; ++m) {
+    const double m0 = mu[p + wq[m][0]], m1 = mu[p + wq[m][1]], m2 = mu[p + wq[m][2]], m3 = mu[p + wq[m][3]];
+    const double hm = harmonic4(m0, m1, m2, m3);
+    if (hm == 0) continue;
+    const double gg = gK(kidx[m]) * kk * c.Hinv;
+    for (int e = 0; e < 4; ++e) {
+      const double me = mu[p + wq[m][e]];
+      const double v = gg * hm * hm / (4 * me * me);
+      if (e == 0) gmu_c += v;
+      else atomicAdd(&c.out[1][g + mq[m][e]], v);
+    }
+  }
+  atomicAdd(&c.out[1][g], gmu_c);
+  // velocity side, per staggering direction
+  const int64_t wstep[3] = {1, wsy, wsz}, mstep[3] = {1, c.msy, c.msz};
+  double A[3], B[3], c1[3], D[3];
+  bool c1zero = false, azero = false;
+  for (int m = 0; m < 3; ++m) {
+    const int64_t p2 = p + wstep[m];
+    A[m] = 0.5 * (rho[p] + rho[p2]);
+    B[m] = 0.5 * (rhof[p] + rhof[p2]);
+    c1[m] = 0.5 * (C1[p] + C1[p2]);
+    const double c2 = 0.5 * (tphi[p] * rhof[p] + tphi[p2] * rhof[p2]);
+    D[m] = (c2 - B[m] * B[m] / A[m]) + 0.5 * c.dt * c1[m];
+    c1zero |= c1[m] == 0;
+    azero |= A[m] == 0;
+  }
+  double grho_c = 0, grf_c = 0;
+  for (int m = 0; m < 3; ++m) {
+    double gA = 0, gB = 0, gc2 = 0;
+    if (!c1zero) {
+      const double gW1 = gK(K_W1X + m), gW2 = gK(K_W2X + m);
+      // W3y multiplies the pressure term of both qy and qz, so its gradient holds both;
+      // W3z is never read
+      const double gW3 = m < 2 ? gK(K_W3X + m) : 0.0;
+      const double D2 = D[m] * D[m];
+      const double dW1_dmp = c.dt * c1[m] / D2;
+      const double dW3_dmp = -k1 * c.Hinv / D2;
+      const double dW2_dmp = -k1 * c.Hinv * (B[m] / A[m]) / D2;
+      const double gmp = gW1 * dW1_dmp + gW2 * dW2_dmp + gW3 * dW3_dmp;
+      gA += gmp * B[m] * B[m] / (A[m] * A[m]) - gW2 * k1 * c.Hinv * B[m] / (A[m] * A[m] * D[m]);
+      gB += gmp * (-2 * B[m] / A[m]) + gW2 * k1 * c.Hinv / (A[m] * D[m]);
+      gc2 += gmp;
+    }
+    if (!azero) {
+      const double gU1 = gK(K_U1X + m), gU2 = gK(K_U2X + m);
+      gA += -gU1 * k1 * c.Hinv / (A[m] * A[m]) - gU2 * B[m] / (A[m] * A[m]);
+      gB += gU2 / A[m];
+    }
+    grho_c += 0.5 * gA;
+    atomicAdd(&c.out[4][g + mstep[m]], 0.5 * gA);
+    grf_c += 0.5 * gB + 0.5 * gc2 * tphi[p];
+    atomicAdd(&c.out[5][g + mstep[m]], 0.5 * gB + 0.5 * gc2 * tphi[p + wstep[m]]);
+  }
+  atomicAdd(&c.out[4][g], grho_c);
+  atomicAdd(&c.out[5][g], grf_c);
+}
+
+// ---- batched state copies ----
+__global__ void __launch_bounds__(256) k_copy_batch(const CopyEntry *tab)
+{
+  const CCopyEntry &e = ((CCopyEntry *)tab)[blockIdx.y];
+  const uint64_t words = e.bytes >> 2;
+  const uint32_t *src = (const uint32_t *)e.src;
+  uint32_t *dst = (uint32_t *)e.dst;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+       w += (uint64_t)gridDim.x * blockDim.x)
+    dst[w] = src[w];
+  const uint32_t rem = (uint32_t)(e.bytes & 3);
+  if (blockIdx.x == 0 && threadIdx.x < rem)
+    ((uint8_t *)e.dst)[(words << 2) + threadIdx.x] = ((const uint8_t *)e.src)[(words << 2) + threadIdx.x];
+}
+
 template <typename S, int ORDER>
-hipError_t adj_launch(int which, const AdjArgs &a, hipStream_t s)
+hipError_t adj_launch(int which, const AdjArgs &a, int q, hipStream_t s)
 {
   const unsigned blocks = (unsigned)((a.nupd + 255) / 256);
   switch (which) {
-    case 0: hipLaunchKernelGGL((k_adj_stress_local<S, ORDER>), dim3(blocks), dim3(256), 0, s, a); break;
-    case 1: hipLaunchKernelGGL((k_adj_stress_gather<S, ORDER>), dim3(blocks), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_adj_velocity_local<S, ORDER>), dim3(blocks), dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_adj_velocity_gather<S, ORDER>), dim3(blocks), dim3(256), 0, s, a); break;
+    case 0: hipLaunchKernelGGL((k_adj<S, ORDER, 0>), dim3(blocks), dim3(256), 0, s, a, q); break;
+    case 1: hipLaunchKernelGGL((k_adj<S, ORDER, 1>), dim3(blocks), dim3(256), 0, s, a, q); break;
+    case 2: hipLaunchKernelGGL((k_adj<S, ORDER, 2>), dim3(blocks), dim3(256), 0, s, a, q); break;
+    case 3: hipLaunchKernelGGL((k_adj<S, ORDER, 3>), dim3(blocks), dim3(256), 0, s, a, q); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+template <typename S, int ORDER>
+hipError_t adj_launch_batch(int which, const AdjBatch &b, int q, hipStream_t s)
+{
+  const unsigned blocks = (unsigned)b.total_blocks;
+  switch (which) {
+    case 0: hipLaunchKernelGGL((k_adj_batch<S, ORDER, 0>), dim3(blocks), dim3(256), 0, s, b, q); break;
+    case 1: hipLaunchKernelGGL((k_adj_batch<S, ORDER, 1>), dim3(blocks), dim3(256), 0, s, b, q); break;
+    case 2: hipLaunchKernelGGL((k_adj_batch<S, ORDER, 2>), dim3(blocks), dim3(256), 0, s, b, q); break;
+    case 3: hipLaunchKernelGGL((k_adj_batch<S, ORDER, 3>), dim3(blocks), dim3(256), 0, s, b, q); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-}  // namespace
+struct SingleRun {
+  const AdjArgs &a;
+  int q;
+  hipStream_t s;
+  template <typename S, int O>
+  hipError_t run(int which) { return adj_launch<S, O>(which, a, q, s); }
+};
+struct BatchRun {
+  const AdjBatch &b;
+  int q;
+  hipStream_t s;
+  template <typename S, int O>
+  hipError_t run(int which) { return adj_launch_batch<S, O>(which, b, q, s); }
+};
 
-hipError_t launch_adjoint_step(int prec, int order, const AdjArgs &a, hipStream_t s)
+// the four passes, in order, for the storage type of prec
+template <class F>
+hipError_t four_passes(int prec, int order, F f)
 {
-  if (a.nupd == 0) return hipSuccess;
+  if (order != 2 && order != 4) return hipErrorInvalidValue;
   for (int which = 0; which < 4; ++which) {
     hipError_t e;
-    if (prec == PREC_FP32) e = order == 2 ? adj_launch<float, 2>(which, a, s) : adj_launch<float, 4>(which, a, s);
-    else if (prec == PREC_FP64) e = order == 2 ? adj_launch<double, 2>(which, a, s) : adj_launch<double, 4>(which, a, s);
+    if (prec == PREC_FP32) e = order == 2 ? f.template run<float, 2>(which) : f.template run<float, 4>(which);
+    else if (prec == PREC_FP64) e = order == 2 ? f.template run<double, 2>(which) : f.template run<double, 4>(which);
+    else if (prec == PREC_FP16) e = order == 2 ? f.template run<_Float16, 2>(which) : f.template run<_Float16, 4>(which);
     else return hipErrorInvalidValue;
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
-hipError_t launch_adjoint_inject(int prec, const InjectArgs &a, hipStream_t s)
+}  // namespace
+
+hipError_t launch_adjoint_step(int prec, int order, const AdjArgs &a, int q, hipStream_t s)
+{
+  if (a.nupd == 0) return hipSuccess;
+  return four_passes(prec, order, SingleRun{a, q, s});
+}
+
+hipError_t launch_adjoint_step_batch(int prec, int order, const AdjBatch &b, int q, hipStream_t s)
+{
+  if (b.total_blocks <= 0) return hipSuccess;
+  return four_passes(prec, order, BatchRun{b, q, s});
+}
+
+hipError_t launch_adjoint_inject(int prec, const InjectArgs &a, int it, hipStream_t s)
 {
   if (a.nrec <= 0 || a.ncomp <= 0) return hipSuccess;
   dim3 grid((unsigned)((a.nrec + 255) / 256), (unsigned)a.ncomp);
-  if (prec == PREC_FP32) hipLaunchKernelGGL(k_adj_inject<float>, grid, dim3(256), 0, s, a);
-  else if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject<double>, grid, dim3(256), 0, s, a);
+  if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject<double>, grid, dim3(256), 0, s, a, it);
+  else if (prec == PREC_FP32 || prec == PREC_FP16)
+    hipLaunchKernelGGL(k_adj_inject<float>, grid, dim3(256), 0, s, a, it);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_inject_batch(int prec, const InjectArgs *tab, int nshots, int64_t max_nrec,
+                                       int max_ncomp, int it, hipStream_t s)
+{
+  if (nshots <= 0 || max_nrec <= 0 || max_ncomp <= 0) return hipSuccess;
+  dim3 grid((unsigned)((max_nrec + 255) / 256), (unsigned)max_ncomp, (unsigned)nshots);
+  if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject_batch<double>, grid, dim3(256), 0, s, tab, it);
+  else if (prec == PREC_FP32 || prec == PREC_FP16)
+    hipLaunchKernelGGL(k_adj_inject_batch<float>, grid, dim3(256), 0, s, tab, it);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -375,6 +614,21 @@ hipError_t launch_misfit(const void *d, bool d_double, const float *obs, float *
     hipLaunchKernelGGL(k_misfit<double>, dim3(blocks), dim3(256), 0, s, (const double *)d, obs, res, n, J);
   else
     hipLaunchKernelGGL(k_misfit<float>, dim3(blocks), dim3(256), 0, s, (const float *)d, obs, res, n, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_chain(const ChainArgs &c, hipStream_t s)
+{
+  const int64_t nw = (int64_t)c.wx * c.wy * c.wz;
+  if (nw <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_chain, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_batch(const CopyEntry *tab, int n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_batch, dim3(1024, (unsigned)n), dim3(256), 0, s, tab);
   return hipGetLastError();
 }
 

@@ -47,7 +47,6 @@ CoefScales coef_scales(const biot_handle *h)
   return CoefScales{(double)h->g.dt, (double)h->g.Cv, (double)h->g.Cp, (double)(1.0f / h->g.dx)};
 }
 
-size_t arith_bytes(int prec) { return prec == BIOT_FP64 ? 8 : 4; }
 
 }  // namespace
 
@@ -72,8 +71,8 @@ extern "C" int biot_set_params(biot_handle *h, const biot_params *p)
 extern "C" int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every)
 {
   if (!h) return BIOT_ERR_ARG;
-  if (h->prec != BIOT_FP32 && h->prec != BIOT_FP64)
-    return afail(h, BIOT_ERR_ARG, "the adjoint path runs on FP32 or FP64 handles");
+  if (h->prec == BIOT_FP16C)
+    return afail(h, BIOT_ERR_ARG, "the adjoint path runs on FP32, FP64 or FP16 handles");
   if (nt <= 0 || every <= 0) return afail(h, BIOT_ERR_ARG, "nt and every must be positive");
   if (!h->have_coef) return afail(h, BIOT_ERR_STATE, "no medium set");
   if (h->nt_max < nt || h->ncomp == 0) return afail(h, BIOT_ERR_STATE, "receivers must record >= nt steps");
@@ -141,94 +140,28 @@ extern "C" int biot_adjoint(biot_handle *h, const float *adj_source, double *con
   if (h->ckpt_nt <= 0) return afail(h, BIOT_ERR_STATE, "biot_adjoint needs biot_forward_checkpointed first");
   if (grad_params && h->par[0].empty())
     return afail(h, BIOT_ERR_STATE, "parameter gradients need biot_set_params");
-  const int nt = h->ckpt_nt, every = h->ckpt_every, order = h->g.order, hh = order / 2;
+  if (grad_params)
+    for (int q = 0; q < 6; ++q)
+      if (!grad_params[q]) return afail(h, BIOT_ERR_ARG, "grad_params[%d] is null", q);
+  const int nt = h->ckpt_nt, every = h->ckpt_every, order = h->g.order;
   const int64_t n = h->n;
-  const size_t asz = arith_bytes(h->prec), sb = h->state_bytes();
-  // ---- buffers
-  size_t lam_bytes = (size_t)NFIELD * n * asz + (size_t)12 * n * asz;
-  for (int ax = 0; ax < 3; ++ax) lam_bytes += (size_t)NCARRY * h->slab[ax] * asz;
   const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
-  if (!h->d_slots && hipMalloc(&h->d_slots, sb * (size_t)(every + 1)) != hipSuccess) {
-    h->d_slots = nullptr;
-    return afail(h, BIOT_ERR_NOMEM, "segment buffer (%d states)", every + 1);
-  }
-  if (!h->d_lam && hipMalloc(&h->d_lam, lam_bytes) != hipSuccess) {
-    h->d_lam = nullptr;
-    return afail(h, BIOT_ERR_NOMEM, "adjoint state");
-  }
-  if (!h->d_grad && hipMalloc((void **)&h->d_grad, (size_t)NKC * n * sizeof(double)) != hipSuccess) {
-    h->d_grad = nullptr;
-    return afail(h, BIOT_ERR_NOMEM, "gradient sums");
-  }
-  if (!h->d_adjsrc && hipMalloc((void **)&h->d_adjsrc, (size_t)m * sizeof(float)) != hipSuccess) {
-    h->d_adjsrc = nullptr;
-    return afail(h, BIOT_ERR_NOMEM, "adjoint source");
-  }
-  AHIP(h, hipMemsetAsync(h->d_lam, 0, lam_bytes, h->stream));
+  int rc = h->alloc_adjoint(every);
+  if (rc) return rc;
+  AHIP(h, hipMemsetAsync(h->d_lam, 0, h->lam_bytes(), h->stream));
   AHIP(h, hipMemsetAsync(h->d_grad, 0, (size_t)NKC * n * sizeof(double), h->stream));
   AHIP(h, hipMemcpyAsync(h->d_adjsrc, adj_source, (size_t)m * sizeof(float), hipMemcpyHostToDevice, h->stream));
 
-  // ---- argument blocks
   AdjArgs A;
-  std::memset(&A, 0, sizeof A);
-  char *lp = (char *)h->d_lam;
-  for (int f = 0; f < NFIELD; ++f) A.lam[f] = lp + (size_t)f * n * asz;
-  lp += (size_t)NFIELD * n * asz;
-  for (int ax = 0; ax < 3; ++ax) {
-    A.lcarry[ax] = lp;
-    lp += (size_t)NCARRY * h->slab[ax] * asz;
-  }
-  for (int d = 0; d < 12; ++d) A.G[d] = lp + (size_t)d * n * asz;
-  for (int k = 0; k < NKC; ++k) {
-    A.grad[k] = h->d_grad + (size_t)k * n;
-    A.k[k] = (const char *)h->d_coef + (size_t)k * n * h->esz;
-  }
-  for (int ax = 0; ax < 3; ++ax) {
-    for (int q = 0; q < 4; ++q) A.prof[ax][q] = h->d_prof[ax][q];
-    A.L[ax] = h->L[ax];
-    A.R[ax] = h->R[ax];
-    A.slab[ax] = h->slab[ax];
-  }
-  A.nx = h->g.nx;
-  A.ny = h->g.ny;
-  A.nz = h->g.nz;
-  A.sy = h->g.nx;
-  A.sz = (int64_t)h->g.nx * h->g.ny;
-  A.nupd = (uint32_t)((int64_t)(h->g.nx - 2 * hh) * (h->g.ny - 2 * hh) * (h->g.nz - 2 * hh));
-  auto slot = [&](int s) { return (char *)h->d_slots + (size_t)s * sb; };
-  auto point_at = [&](int s, const void **fields, const void **carry) {
-    char *p = slot(s);
-    for (int f = 0; f < NFIELD; ++f) fields[f] = p + (size_t)f * n * h->esz;
-    p += (size_t)NFIELD * n * h->esz;
-    for (int ax = 0; ax < 3; ++ax) {
-      if (carry) carry[ax] = p;
-      p += (size_t)NCARRY * h->slab[ax] * h->csz;
-    }
-  };
+  h->adj_args(A);
   InjectArgs I;
-  std::memset(&I, 0, sizeof I);
-  for (int f = 0; f < NFIELD; ++f) I.lam[f] = A.lam[f];
-  I.rec = (const int64_t *)h->d_rec;
-  I.nrec = h->nrec;
-  I.ncomp = h->ncomp;
-  for (int c = 0; c < h->ncomp; ++c) {
-    I.comp[c] = h->comp[c];
-    I.scale[c] = h->fscale[h->comp[c]];  // traces hold scale x stored value
-  }
-  I.src = h->d_adjsrc;
-  I.nt = h->nt_max;
+  h->inject_args(I);
   SourceArgs sa;
-  std::memset(&sa, 0, sizeof sa);
-  for (int f = 0; f < NFIELD; ++f) {
-    sa.f[f] = h->field_ptr(f);
-    sa.lo[f] = h->lo_ptr(f);
-    sa.inv_scale[f] = 1.0f / h->fscale[f];
-  }
-  sa.cell = h->src_cell;
-  sa.mask = h->src_mask;
-  sa.wavelet = (const float *)h->d_wavelet;
+  h->source_args(sa);
   StepArgs st;
   h->fill_args(st);
+  const size_t sb = h->state_bytes();
+  auto slot = [&](int q) { return (char *)h->d_slots + (size_t)q * sb; };
 
   // ---- backward sweep over the checkpoint segments
   AHIP(h, hipEventRecord(h->ev0, h->stream));
@@ -248,11 +181,8 @@ extern "C" int biot_adjoint(biot_handle *h, const float *adj_source, double *con
     }
     h->save_state(slot(K));
     for (int it = n0 + K - 1; it >= n0; --it) {
-      I.it = it;
-      AHIP(h, launch_adjoint_inject(h->prec, I, h->stream));
-      point_at(it - n0, A.fw, A.fcarry);
-      point_at(it - n0 + 1, A.fw1, nullptr);
-      AHIP(h, launch_adjoint_step(h->prec, order, A, h->stream));
+      AHIP(h, launch_adjoint_inject(h->prec, I, it, h->stream));
+      AHIP(h, launch_adjoint_step(h->prec, order, A, it - n0, h->stream));
     }
   }
   AHIP(h, hipEventRecord(h->ev1, h->stream));
@@ -261,17 +191,18 @@ extern "C" int biot_adjoint(biot_handle *h, const float *adj_source, double *con
   AHIP(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
 
-  // ---- gradients out
+  // ---- gradients out: storage-unit coefficient gradients times the per-array factor
   std::vector<double> gK((size_t)NKC * n);
   AHIP(h, hipMemcpy(gK.data(), h->d_grad, gK.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < NKC; ++k)
+    if (h->kfac[k] != 1.0)
+      for (int64_t c = 0; c < n; ++c) gK[(size_t)k * n + c] *= h->kfac[k];
   if (grad_coef)
     for (int k = 0; k < NKC; ++k)
       if (grad_coef[k]) std::memcpy(grad_coef[k], gK.data() + (size_t)k * n, (size_t)n * sizeof(double));
   if (grad_params) {
     const double *gk[NKC];
     for (int k = 0; k < NKC; ++k) gk[k] = gK.data() + (size_t)k * n;
-    for (int q = 0; q < 6; ++q)
-      if (!grad_params[q]) return afail(h, BIOT_ERR_ARG, "grad_params[%d] is null", q);
     for (int q = 0; q < 6; ++q) std::memset(grad_params[q], 0, (size_t)n * sizeof(double));
     chain_params(h->g.nx, h->g.ny, h->g.nz, coef_scales(h), param_view(h), gk, grad_params);
   }

@@ -63,12 +63,20 @@ inline size_t carry_bytes(int prec) { return prec == PREC_FP64 ? 8 : 4; }
 // Host-side launchers (biot_kernels.hip).
 struct LaunchShape {
   int bx, by, zchunk;
+  int xv = 1;  // cells per lane along x (1: scalar kernels; 2, 4: x-vectorised kernels)
 };
 
 hipError_t launch_velocity(int prec, int order, const StepArgs &args, LaunchShape shape,
                            hipStream_t s);
 hipError_t launch_stress(int prec, int order, const StepArgs &args, LaunchShape shape,
                          hipStream_t s);
+// x-vectorised kernels (biot_kernels_xv.hip): FP32 and FP16 with xv 2 or 4, FP16C with 4;
+// nx must be a multiple of 4
+bool xv_supported(int prec, int xv, int nx);
+hipError_t launch_velocity_xv(int prec, int order, const StepArgs &args, LaunchShape shape,
+                              hipStream_t s);
+hipError_t launch_stress_xv(int prec, int order, const StepArgs &args, LaunchShape shape,
+                            hipStream_t s);
 struct SourceArgs {
   void *f[NFIELD];
   int8_t *lo[NFIELD];
@@ -76,6 +84,7 @@ struct SourceArgs {
   uint32_t mask;
   const float *wavelet;     // device, one sample per step
   int it;
+  int nt_w;                 // wavelet samples (batched launches check the step against it)
   float inv_scale[NFIELD];  // 1 / storage scale per field
 };
 struct RecordArgs {
@@ -91,6 +100,23 @@ struct RecordArgs {
   int it, nt_max;
 };
 hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s);
+
+// Batched launches over many shots (one grid for all; device-resident argument tables).
+struct BatchArgs {
+  const StepArgs *shots;    // device, one per shot (gx, gy, gz, zchunk set for the shape)
+  const int *tile_start;    // device, nshots + 1 prefix sums of the shots' tile counts
+  int nshots, total_tiles;
+};
+// blocks: the persistent grid size (a multiple of 8)
+hipError_t launch_step_batch(int prec, int order, bool vel, const BatchArgs &b, LaunchShape sh,
+                             int blocks, hipStream_t s);
+hipError_t launch_source_batch(int prec, const SourceArgs *tab, int nshots, const int *it,
+                               hipStream_t s);
+hipError_t launch_record_batch(int prec, const RecordArgs *tab, int nshots, int64_t max_nrec,
+                               int max_ncomp, const int *it, hipStream_t s);
+// *it += 1 / *it = v on the device (kernels, so they may sit in a captured graph)
+hipError_t launch_advance(int *it, hipStream_t s);
+hipError_t launch_set_counter(int *it, int v, hipStream_t s);
 // multiply a stored array by f (lo: its FP16C residuals or null); carries use prec of the
 // carry type, i.e. pass PREC_FP32 for FP16/FP16C carries
 hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hipStream_t s);

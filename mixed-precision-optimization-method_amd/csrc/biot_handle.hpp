@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/biot.h"
+#include "biot_adjoint.hpp"
 #include "biot_device.hpp"
 
 struct biot_handle {
@@ -34,6 +35,7 @@ struct biot_handle {
   std::vector<float> prof[3][4];            // d, d2, e, e2 per axis (before the x0.01 pack)
   bool have_coef = false;
   float ks[biot::NKC];        // per-coefficient-array scale (binary16 storage)
+  double kfac[biot::NKC];     // storage-unit coefficient / physical coefficient, per array
   float fscale[biot::NFIELD]; // per-field storage scale
 
   // source
@@ -75,12 +77,21 @@ struct biot_handle {
   double *d_grad = nullptr;                  // 22 FP64 coefficient gradients
   float *d_adjsrc = nullptr;                 // adjoint source traces
   size_t state_bytes() const;
+  size_t state_off(int piece) const;  // 0 fields, 1..3 carries of x, y, z, 4 total
   void save_state(void *dst);
   void load_state(const void *src);
   void free_adjoint();
   void forget_model();  // drop parameter-built coefficients and every adjoint buffer
   // marks: null, or 3 events per step recorded before velocity, between, after stress
   int enqueue_steps(int nsteps, hipEvent_t *marks);
+
+  // argument blocks shared by the single-grid and batched drivers
+  void source_args(biot::SourceArgs &sa) const;
+  void record_args(biot::RecordArgs &ra) const;
+  size_t lam_bytes() const;                  // adjoint wavefields, carries, derivative adjoints
+  int alloc_adjoint(int every);              // d_slots (every+1 states), d_lam, d_grad, d_adjsrc
+  void adj_args(biot::AdjArgs &A) const;     // after alloc_adjoint
+  void inject_args(biot::InjectArgs &I) const;
 };
 
 biot::LaunchShape default_shape(int prec, int order);

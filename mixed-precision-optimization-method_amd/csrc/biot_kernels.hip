@@ -99,24 +99,24 @@ __device__ __forceinline__ void store_raw(void *f, int8_t *lo, bool comp_f, u32 
     }
   }
 }
-template <class P>
-__device__ __forceinline__ typename P::A load_rt(const StepArgs &a, int f, u32 idx)
+template <class P, class AT>
+__device__ __forceinline__ typename P::A load_rt(const AT &a, int f, u32 idx)
 {
   return load_raw<P>(a.f[f], a.lo[f], (COMP_FIELDS >> f) & 1u, idx);
 }
-template <class P>
-__device__ __forceinline__ void store_rt(const StepArgs &a, int f, u32 idx, typename P::A x)
+template <class P, class AT>
+__device__ __forceinline__ void store_rt(const AT &a, int f, u32 idx, typename P::A x)
 {
   store_raw<P>(a.f[f], a.lo[f], (COMP_FIELDS >> f) & 1u, idx, x);
 }
 // compile-time field index: the residual test folds away
-template <class P, int F>
-__device__ __forceinline__ typename P::A L(const StepArgs &a, u32 idx)
+template <class P, int F, class AT>
+__device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
 {
   return load_rt<P>(a, F, idx);
 }
-template <class P, int F>
-__device__ __forceinline__ void ST(const StepArgs &a, u32 idx, typename P::A x)
+template <class P, int F, class AT>
+__device__ __forceinline__ void ST(const AT &a, u32 idx, typename P::A x)
 {
   store_rt<P>(a, F, idx, x);
 }
@@ -133,8 +133,8 @@ __device__ __forceinline__ A dwin(const A *w)
 }
 
 // forward difference of field F at c + s/2 (value at c supplied)
-template <class P, int F, int ORDER>
-__device__ __forceinline__ typename P::A dfw(const StepArgs &a, u32 c, u32 s, typename P::A fc)
+template <class P, int F, int ORDER, class AT>
+__device__ __forceinline__ typename P::A dfw(const AT &a, u32 c, u32 s, typename P::A fc)
 {
   using A = typename P::A;
   if constexpr (ORDER == 2) {
@@ -145,8 +145,8 @@ __device__ __forceinline__ typename P::A dfw(const StepArgs &a, u32 c, u32 s, ty
   }
 }
 // backward difference of field F at c - s/2 (value at c supplied)
-template <class P, int F, int ORDER>
-__device__ __forceinline__ typename P::A dbw(const StepArgs &a, u32 c, u32 s, typename P::A fc)
+template <class P, int F, int ORDER, class AT>
+__device__ __forceinline__ typename P::A dbw(const AT &a, u32 c, u32 s, typename P::A fc)
 {
   using A = typename P::A;
   if constexpr (ORDER == 2) {
@@ -157,8 +157,8 @@ __device__ __forceinline__ typename P::A dbw(const StepArgs &a, u32 c, u32 s, ty
   }
 }
 
-template <class P>
-__device__ __forceinline__ typename P::A coef(const StepArgs &a, int idx, u32 c)
+template <class P, class AT>
+__device__ __forceinline__ typename P::A coef(const AT &a, int idx, u32 c)
 {
   using A = typename P::A;
   const A v = (A)at<typename P::S>(a.k[idx], c);
@@ -212,14 +212,13 @@ __device__ __forceinline__ A prof(const void *p, int i)
 
 enum { FVX = 0, FVY, FVZ, FQX, FQY, FQZ, FTXX, FTYY, FTZZ, FTXY, FTXZ, FTYZ, FP };
 
-template <class P, int ORDER>
-__global__ void __launch_bounds__(256) k_velocity(const StepArgs a)
+// one x-y tile of the update box, planes kb..ke-1 of z chunk tz
+template <class P, int ORDER, class AT>
+__device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int tz)
 {
   using A = typename P::A;
   constexpr int h = ORDER / 2;
   constexpr int W = 2 * h;
-  int tx, ty, tz;
-  if (!tile_of(a, tx, ty, tz)) return;
   const int i = tx * blockDim.x + threadIdx.x;
   const int j = ty * blockDim.y + threadIdx.y;
   if (i < h || i >= a.nx - h || j < h || j >= a.ny - h) return;
@@ -339,14 +338,13 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs a)
   }
 }
 
-template <class P, int ORDER>
-__global__ void __launch_bounds__(256) k_stress(const StepArgs a)
+// one x-y tile of the update box, planes kb..ke-1 of z chunk tz
+template <class P, int ORDER, class AT>
+__device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
 {
   using A = typename P::A;
   constexpr int h = ORDER / 2;
   constexpr int W = 2 * h;
-  int tx, ty, tz;
-  if (!tile_of(a, tx, ty, tz)) return;
   const int i = tx * blockDim.x + threadIdx.x;
   const int j = ty * blockDim.y + threadIdx.y;
   if (i < h || i >= a.nx - h || j < h || j >= a.ny - h) return;
@@ -456,33 +454,119 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs a)
   }
 }
 
+template <class P, int ORDER>
+__global__ void __launch_bounds__(256) k_velocity(const StepArgs a)
+{
+  int tx, ty, tz;
+  if (!tile_of(a, tx, ty, tz)) return;
+  velocity_tile<P, ORDER>(a, tx, ty, tz);
+}
+template <class P, int ORDER>
+__global__ void __launch_bounds__(256) k_stress(const StepArgs a)
+{
+  int tx, ty, tz;
+  if (!tile_of(a, tx, ty, tz)) return;
+  stress_tile<P, ORDER>(a, tx, ty, tz);
+}
+
+// Batched forms: one persistent grid walks the tiles of every shot.  The shots' tile
+// lists are concatenated (tile_start[s] = first global tile of shot s); each XCD takes one
+// contiguous range of that list, so a shot's neighbouring tiles share an L2, and a block
+// strides through its XCD's range until it is exhausted (every wave exits).
+// The argument tables are read-only for a launch and indexed uniformly per block: read
+// them through the constant address space, so the compiler keeps them in scalar registers
+// (as it does kernel arguments) instead of a VGPR pair per pointer.
+typedef __attribute__((address_space(4))) const StepArgs CStepArgs;
+typedef __attribute__((address_space(4))) const int CInt;
+__device__ __forceinline__ int shot_of(const BatchArgs &b, int t)
+{
+  CInt *ts = ((CInt *)b.tile_start);
+  int lo = 0, hi = b.nshots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ts[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+template <class P, int ORDER, bool VEL>
+__global__ void __launch_bounds__(256) k_step_batch(const BatchArgs b)
+{
+  const int x = blockIdx.x & 7, l = blockIdx.x >> 3, per_x = gridDim.x >> 3;
+  const int per = (b.total_tiles + 7) >> 3;
+  const int end = min((x + 1) * per, b.total_tiles);
+  for (int t = x * per + l; t < end; t += per_x) {
+    const int s = shot_of(b, t);
+    const CStepArgs &a = ((CStepArgs *)b.shots)[s];
+    const int lt = t - ((CInt *)b.tile_start)[s];
+    const int tx = lt % a.gx, r = lt / a.gx, ty = r % a.gy, tz = r / a.gy;
+    if (VEL) velocity_tile<P, ORDER>(a, tx, ty, tz);
+    else stress_tile<P, ORDER>(a, tx, ty, tz);
+  }
+}
+
 // Source injection (kernel.cu:130-142) for one cell: one lane per field.
 template <class P>
-__global__ void k_source(const SourceArgs sa)
+__device__ __forceinline__ void source_cell(const SourceArgs &sa, int it)
 {
   using A = typename P::A;
   const int f = threadIdx.x;
   if (f >= NFIELD || !(sa.mask & (1u << f))) return;
   const u32 c = (u32)sa.cell;
   const bool cf = (COMP_FIELDS >> f) & 1u;
-  const A amp = (A)sa.wavelet[sa.it] * (A)sa.inv_scale[f];
+  const A amp = (A)sa.wavelet[it] * (A)sa.inv_scale[f];
   store_raw<P>(sa.f[f], sa.lo[f], cf, c, load_raw<P>(sa.f[f], sa.lo[f], cf, c) + amp);
+}
+template <class P>
+__global__ void k_source(const SourceArgs sa)
+{
+  source_cell<P>(sa, sa.it);
 }
 
 // Receiver sampling: traces[comp][it][rec] = field value * storage scale.
 template <class P>
-__global__ void k_record(const RecordArgs ra)
+__device__ __forceinline__ void record_one(const RecordArgs &ra, int64_t r, int ci, int it)
 {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int ci = blockIdx.y;
   if (r >= ra.nrec || ci >= ra.ncomp) return;
   const int f = ra.comp[ci];
   // the product of two floats is exact in double, so (float)v is the rounded float product
   const double v = (double)load_raw<P>(ra.f[f], ra.lo[f], (COMP_FIELDS >> f) & 1u, (u32)ra.rec[r]) *
                    (double)ra.scale[ci];
-  const int64_t o = ((int64_t)ci * ra.nt_max + ra.it) * ra.nrec + r;
+  const int64_t o = ((int64_t)ci * ra.nt_max + it) * ra.nrec + r;
   ra.traces[o] = (float)v;
   if (ra.traces64) ra.traces64[o] = v;
+}
+template <class P>
+__global__ void k_record(const RecordArgs ra)
+{
+  record_one<P>(ra, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.y, ra.it);
+}
+
+// Batched source / record: one block (z) per shot; the step index comes from a device
+// counter that k_advance moves on, so a captured sequence of steps can be replayed.
+template <class P>
+__global__ void k_source_batch(const SourceArgs *tab, const int *itp)
+{
+  const SourceArgs &sa = tab[blockIdx.x];
+  const int it = *itp;
+  if (!sa.wavelet || it >= sa.nt_w) return;
+  source_cell<P>(sa, it);
+}
+template <class P>
+__global__ void k_record_batch(const RecordArgs *tab, const int *itp)
+{
+  const RecordArgs &ra = tab[blockIdx.z];
+  const int it = *itp;
+  if (!ra.traces || it >= ra.nt_max) return;
+  record_one<P>(ra, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.y, it);
+}
+__global__ void k_advance(int *itp)
+{
+  if (threadIdx.x == 0) atomicAdd(itp, 1);
+}
+__global__ void k_set_counter(int *itp, int v)
+{
+  if (threadIdx.x == 0) atomicExch(itp, v);
 }
 
 // In-place multiply of a stored array (re-expresses values when a storage scale changes).
@@ -551,12 +635,83 @@ hipError_t dispatch(int prec, int order, const StepArgs &a, LaunchShape sh, hipS
 
 }  // namespace
 
+hipError_t launch_step_batch(int prec, int order, bool vel, const BatchArgs &b, LaunchShape sh,
+                             int blocks, hipStream_t s)
+{
+  if (sh.bx * sh.by > 256 || sh.bx <= 0 || sh.by <= 0 || blocks <= 0 || blocks % 8) return hipErrorInvalidValue;
+  if (b.total_tiles <= 0) return hipSuccess;
+  dim3 grid((unsigned)blocks), block(sh.bx, sh.by, 1);
+#define BIOT_BATCH_CASE(PP, OO)                                                                 \
+  if (vel) hipLaunchKernelGGL((k_step_batch<PP, OO, true>), grid, block, 0, s, b);              \
+  else hipLaunchKernelGGL((k_step_batch<PP, OO, false>), grid, block, 0, s, b);
+  if (order != 2 && order != 4) return hipErrorInvalidValue;
+  switch (prec) {
+    case PREC_FP32:
+      if (order == 2) { BIOT_BATCH_CASE(P32, 2) } else { BIOT_BATCH_CASE(P32, 4) }
+      break;
+    case PREC_FP16:
+      if (order == 2) { BIOT_BATCH_CASE(P16, 2) } else { BIOT_BATCH_CASE(P16, 4) }
+      break;
+    case PREC_FP16C:
+      if (order == 2) { BIOT_BATCH_CASE(P16C, 2) } else { BIOT_BATCH_CASE(P16C, 4) }
+      break;
+    case PREC_FP64:
+      if (order == 2) { BIOT_BATCH_CASE(P64, 2) } else { BIOT_BATCH_CASE(P64, 4) }
+      break;
+    default: return hipErrorInvalidValue;
+  }
+#undef BIOT_BATCH_CASE
+  return hipGetLastError();
+}
+
+hipError_t launch_source_batch(int prec, const SourceArgs *tab, int nshots, const int *it, hipStream_t s)
+{
+  if (nshots <= 0) return hipSuccess;
+  switch (prec) {
+    case PREC_FP32: hipLaunchKernelGGL(k_source_batch<P32>, dim3(nshots), dim3(64), 0, s, tab, it); break;
+    case PREC_FP16: hipLaunchKernelGGL(k_source_batch<P16>, dim3(nshots), dim3(64), 0, s, tab, it); break;
+    case PREC_FP16C: hipLaunchKernelGGL(k_source_batch<P16C>, dim3(nshots), dim3(64), 0, s, tab, it); break;
+    case PREC_FP64: hipLaunchKernelGGL(k_source_batch<P64>, dim3(nshots), dim3(64), 0, s, tab, it); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_record_batch(int prec, const RecordArgs *tab, int nshots, int64_t max_nrec,
+                               int max_ncomp, const int *it, hipStream_t s)
+{
+  if (nshots <= 0 || max_nrec <= 0 || max_ncomp <= 0) return hipSuccess;
+  dim3 grid((unsigned)((max_nrec + 255) / 256), (unsigned)max_ncomp, (unsigned)nshots), block(256);
+  switch (prec) {
+    case PREC_FP32: hipLaunchKernelGGL(k_record_batch<P32>, grid, block, 0, s, tab, it); break;
+    case PREC_FP16: hipLaunchKernelGGL(k_record_batch<P16>, grid, block, 0, s, tab, it); break;
+    case PREC_FP16C: hipLaunchKernelGGL(k_record_batch<P16C>, grid, block, 0, s, tab, it); break;
+    case PREC_FP64: hipLaunchKernelGGL(k_record_batch<P64>, grid, block, 0, s, tab, it); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(int *it, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, it);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_counter(int *it, int v, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, s, it, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_velocity(int prec, int order, const StepArgs &args, LaunchShape sh, hipStream_t s)
 {
+  if (sh.xv > 1) return launch_velocity_xv(prec, order, args, sh, s);
   return dispatch<true>(prec, order, args, sh, s);
 }
 hipError_t launch_stress(int prec, int order, const StepArgs &args, LaunchShape sh, hipStream_t s)
 {
+  if (sh.xv > 1) return launch_stress_xv(prec, order, args, sh, s);
   return dispatch<false>(prec, order, args, sh, s);
 }
 
