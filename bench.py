@@ -181,6 +181,44 @@ def adjoint_shot(repeats=2):
     return best
 
 
+def batch_shots(nt=1000, every=50):
+    """16 shots in cubic windows of 96..192 cells (seeded) cut from the 512 x 512 x 384
+    perturbed medium, FP16 wavefields: the graph-captured batched forward (throughput over
+    all windows) and the batched misfit gradient (checkpoints every 50 steps, FP64 sums,
+    per-shot gradients added into the model grid on the device)."""
+    import numpy as np
+
+    from biot_amd import configs
+
+    t0 = time.time()
+    cfg = configs.batch16(nt=nt)
+    b = configs.make_batch(cfg, precision="fp16")
+    cells = sum(int(np.prod(sh["size"])) for sh in cfg["shots"])
+    log(f"[bench] batch: {len(cfg['shots'])} shots, {cells / 1e6:.1f} M cells, set-up {time.time() - t0:.1f} s")
+    launch = b.autotune(3)
+    b.forward(nt, graph_steps=50)
+    obs = [b.traces(i) for i in range(len(cfg["shots"]))]
+    b.forward(nt, graph_steps=50)
+    f_ms = b.last_ms()
+    # observed data: the same shots with a slower, softer start model would need a second
+    # model; the gradient's cost does not depend on the data, so the recorded traces scaled
+    # by 1.01 stand in for observations
+    obs = [o * np.float32(1.01) for o in obs]
+    J, g = b.gradient(nt, obs, every=every)
+    g_ms = b.last_ms()
+    groups = b.gradient_groups()
+    finite = bool(all(np.isfinite(v).all() for v in g.values()))
+    b.close()
+    return dict(workload=f"batch16: 16 shots, cubic windows {min(sh['size'][0] for sh in cfg['shots'])}.."
+                         f"{max(sh['size'][0] for sh in cfg['shots'])} cells cut from the 512x512x384 "
+                         f"perturbed medium, order 4, PML 20, {nt} steps, FP16 wavefields",
+                shots=len(cfg["shots"]), cells=cells, nt=nt,
+                forward_ms=round(f_ms, 1), forward_Mcell_updates_per_s=round(cells * nt / f_ms / 1e3, 1),
+                graph_steps=50, launch_bx_by_zchunk_blocks=list(launch),
+                gradient_ms=round(g_ms, 1), gradient_ms_per_shot=round(g_ms / len(cfg["shots"]), 1),
+                checkpoint_every=every, gradient_groups=groups, misfit=J, finite=finite)
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB (torch's copy kernel), repeated 5 times:
     the HBM-traffic counters of this dispatch calibrate FETCH_SIZE / WRITE_SIZE."""
@@ -206,6 +244,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.copy_calib:
         copy_calibration(args.copy_calib)
@@ -280,6 +319,11 @@ def main():
         log("[bench] adjoint 128^3 x 1000 steps ...")
         out["adjoint"] = adjoint_shot()
         log(f"[bench] adjoint: {out['adjoint']['ms_per_shot']:.0f} ms/shot")
+    if not args.no_batch:
+        log("[bench] batched shots ...")
+        out["batch"] = batch_shots()
+        log(f"[bench] batch: forward {out['batch']['forward_Mcell_updates_per_s']:.0f} Mcell/s, "
+            f"gradient {out['batch']['gradient_ms']:.0f} ms")
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)

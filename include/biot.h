@@ -235,6 +235,9 @@ int biot_batch_read_traces(biot_batch *b, int32_t shot, float *out, int64_t coun
  * through checkpoints every `every` steps and the adjoint of every shot at once */
 int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, const float *const observed[],
                         double *J, double *const grad[6]);
+/* shots are processed in groups that fit the free device memory (all at once when they
+ * fit); the number of groups the last biot_batch_gradient used */
+int biot_batch_gradient_groups(biot_batch *b, int32_t *groups);
 /* HIP-event time of the last biot_batch_forward or biot_batch_gradient */
 int biot_batch_last_ms(biot_batch *b, double *ms);
 /* frees the gradient's checkpoints and adjoint buffers */

@@ -154,6 +154,7 @@ def lib():
         "biot_batch_read_traces": ([H, i32, FP, i64], C.c_int),
         "biot_batch_gradient": ([H, i32, i32, C.POINTER(FP), _DP, C.POINTER(_DP)], C.c_int),
         "biot_batch_last_ms": ([H, _DP], C.c_int),
+        "biot_batch_gradient_groups": ([H, C.POINTER(i32)], C.c_int),
         "biot_batch_release": ([H], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -182,6 +183,7 @@ def exported_symbols():
             "biot_batch_shot_count", "biot_batch_shot_handle", "biot_batch_set_launch",
             "biot_batch_get_launch", "biot_batch_autotune", "biot_batch_forward",
             "biot_batch_read_traces", "biot_batch_gradient", "biot_batch_last_ms",
+            "biot_batch_gradient_groups",
             "biot_batch_release"]
 
 

@@ -106,5 +106,10 @@ class Batch:
         self._check(lib().biot_batch_last_ms(self._h, C.byref(ms)), "biot_batch_last_ms")
         return ms.value
 
+    def gradient_groups(self):
+        g = C.c_int32()
+        self._check(lib().biot_batch_gradient_groups(self._h, C.byref(g)), "biot_batch_gradient_groups")
+        return g.value
+
     def release(self):
         self._check(lib().biot_batch_release(self._h), "biot_batch_release")

@@ -48,6 +48,7 @@ struct biot_batch {
   RecordArgs *d_rec = nullptr;
   int *d_it = nullptr;
   int total_tiles = 0;
+  int nactive = 0;                 // shots the current tables cover
   int64_t max_nrec = 0;
   int max_ncomp = 0;
   bool built = false;
@@ -63,6 +64,7 @@ struct biot_batch {
   int adj_blocks = 0;
   double *d_gout = nullptr;        // 6 model-grid arrays
   double *d_J = nullptr;
+  int groups = 0;                  // shot groups of the last gradient (device memory)
 
   ~biot_batch();
   void drop_graph()
@@ -237,12 +239,16 @@ extern "C" int biot_batch_shot_count(biot_batch *b, int32_t *n)
 
 namespace {
 
-// device tables of the forward launches for the current shape
-int build_tables(biot_batch *b)
+// device tables of the forward launches for the current shape, for shots [s0, s1) (all
+// shots when s1 < 0); the tables hold the subset from their first entry
+int build_tables(biot_batch *b, int s0 = 0, int s1 = -1)
 {
-  if (b->built) return BIOT_OK;
-  const int S = (int)b->shots.size();
-  if (S == 0) return bfail(b, BIOT_ERR_STATE, "no shots");
+  const bool all = s1 < 0;
+  if (all && b->built) return BIOT_OK;
+  if (all) s1 = (int)b->shots.size();
+  const int S = s1 - s0;
+  if (S <= 0) return bfail(b, BIOT_ERR_STATE, "no shots");
+  const int Sall = (int)b->shots.size();
   std::vector<StepArgs> st(S);
   std::vector<int> tiles(S + 1, 0);
   std::vector<SourceArgs> src(S);
@@ -251,7 +257,7 @@ int build_tables(biot_batch *b)
   b->max_ncomp = 0;
   const int h2 = b->cfg.order / 2;
   for (int i = 0; i < S; ++i) {
-    biot_handle *h = b->shots[i].h;
+    biot_handle *h = b->shots[s0 + i].h;
     StepArgs &a = st[i];
     h->fill_args(a);
     a.zchunk = b->shape.zchunk;
@@ -266,29 +272,31 @@ int build_tables(biot_batch *b)
     b->max_ncomp = std::max(b->max_ncomp, h->ncomp);
   }
   b->total_tiles = tiles[S];
+  b->nactive = S;
   if (!b->d_step) {
-    BHIP(b, hipMalloc((void **)&b->d_step, S * sizeof(StepArgs)));
-    BHIP(b, hipMalloc((void **)&b->d_tiles, (S + 1) * sizeof(int)));
-    BHIP(b, hipMalloc((void **)&b->d_src, S * sizeof(SourceArgs)));
-    BHIP(b, hipMalloc((void **)&b->d_rec, S * sizeof(RecordArgs)));
+    BHIP(b, hipMalloc((void **)&b->d_step, Sall * sizeof(StepArgs)));
+    BHIP(b, hipMalloc((void **)&b->d_tiles, (Sall + 1) * sizeof(int)));
+    BHIP(b, hipMalloc((void **)&b->d_src, Sall * sizeof(SourceArgs)));
+    BHIP(b, hipMalloc((void **)&b->d_rec, Sall * sizeof(RecordArgs)));
   }
   BHIP(b, hipMemcpy(b->d_step, st.data(), S * sizeof(StepArgs), hipMemcpyHostToDevice));
   BHIP(b, hipMemcpy(b->d_tiles, tiles.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
   BHIP(b, hipMemcpy(b->d_src, src.data(), S * sizeof(SourceArgs), hipMemcpyHostToDevice));
   BHIP(b, hipMemcpy(b->d_rec, rec.data(), S * sizeof(RecordArgs), hipMemcpyHostToDevice));
-  b->built = true;
+  b->built = all;
+  if (!all) b->drop_graph();  // a captured graph refers to the full tables
   return BIOT_OK;
 }
 
 BatchArgs batch_args(const biot_batch *b)
 {
-  return BatchArgs{b->d_step, b->d_tiles, (int)b->shots.size(), b->total_tiles};
+  return BatchArgs{b->d_step, b->d_tiles, b->nactive, b->total_tiles};
 }
 
 // one step of every shot: source, velocity, stress, (record), counter + 1
 int enqueue_step(biot_batch *b, bool record)
 {
-  const int S = (int)b->shots.size();
+  const int S = b->nactive;
   const int prec = b->cfg.precision, order = b->cfg.order;
   const BatchArgs ba = batch_args(b);
   BHIP(b, launch_source_batch(prec, b->d_src, S, b->d_it, b->stream));
@@ -465,37 +473,34 @@ extern "C" int biot_batch_shot_handle(biot_batch *b, int32_t shot, biot_handle *
   return BIOT_OK;
 }
 
-extern "C" int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, const float *const observed[],
-                                   double *J, double *const grad[6])
+namespace {
+
+// device bytes the gradient needs for one shot
+size_t shot_gradient_bytes(const biot_handle *h, int nseg, int every)
 {
-  if (!b) return BIOT_ERR_ARG;
-  if (!observed || !J || !grad) return bfail(b, BIOT_ERR_ARG, "observed, J and grad are required");
-  for (int q = 0; q < 6; ++q)
-    if (!grad[q]) return bfail(b, BIOT_ERR_ARG, "grad[%d] is null", q);
-  const int S = (int)b->shots.size();
-  if (S == 0) return bfail(b, BIOT_ERR_STATE, "no shots");
-  if (nt <= 0 || every <= 0) return bfail(b, BIOT_ERR_ARG, "nt and every must be positive");
-  for (int i = 0; i < S; ++i) {
-    if (!observed[i]) return bfail(b, BIOT_ERR_ARG, "observed[%d] is null", i);
-    if (b->shots[i].h->nt_max < nt) return bfail(b, BIOT_ERR_ARG, "shot %d records fewer than nt steps", i);
-    if (b->shots[i].h->src_mask & 0x3Fu)
-      return bfail(b, BIOT_ERR_ARG, "shot %d: the adjoint path supports sources on stresses and pressure only", i);
-  }
+  const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
+  return h->state_bytes() * (size_t)(nseg + every + 1) + h->lam_bytes() +
+         (size_t)NKC * h->n * sizeof(double) + (size_t)8 * h->n * sizeof(double) +
+         (size_t)2 * std::max<int64_t>(m, 1) * sizeof(float);
+}
+
+// the gradient of shots [s0, s1): checkpointed forward, misfit, recompute + adjoint, chain
+int gradient_group(biot_batch *b, int s0, int s1, int nt, int every, const float *const observed[])
+{
+  const int S = s1 - s0;
   const int prec = b->cfg.precision, order = b->cfg.order;
   const int nseg = (nt + every - 1) / every;
   const int64_t N = (int64_t)b->cfg.nx * b->cfg.ny * b->cfg.nz;
-  int rc = build_tables(b);
+  int rc = build_tables(b, s0, s1);
   if (rc) return rc;
-
-  // ---- buffers (kept between calls with the same every / nt)
-  b->free_gradient();
-  for (auto &s : b->shots) {
+  for (int i = s0; i < s1; ++i) {
+    auto &s = b->shots[i];
     biot_handle *h = s.h;
     if ((rc = h->alloc_adjoint(every))) return hfail(b, h, rc, "adjoint buffers");
     for (int k = 0; k < nseg; ++k) {
       void *p = nullptr;
       if (hipMalloc(&p, h->state_bytes()) != hipSuccess)
-        return bfail(b, BIOT_ERR_NOMEM, "checkpoints of %d shots x %d segments do not fit", S, nseg);
+        return bfail(b, BIOT_ERR_NOMEM, "checkpoints of shot %d do not fit", i);
       s.ckpt.push_back(p);
     }
     for (int q = 0; q < 8; ++q) {
@@ -504,18 +509,16 @@ extern "C" int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, con
     }
     const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
     BHIP(b, hipMalloc((void **)&s.d_obs, (size_t)std::max<int64_t>(m, 1) * sizeof(float)));
+    BHIP(b, hipMemcpy(s.d_obs, observed[i], (size_t)m * sizeof(float), hipMemcpyHostToDevice));
   }
-  BHIP(b, hipMalloc((void **)&b->d_gout, (size_t)6 * N * sizeof(double)));
-  BHIP(b, hipMalloc((void **)&b->d_J, sizeof(double)));
-
   // adjoint / inject tables
   {
     std::vector<AdjArgs> adj(S);
     std::vector<InjectArgs> inj(S);
     std::vector<int> bs(S + 1, 0);
     for (int i = 0; i < S; ++i) {
-      b->shots[i].h->adj_args(adj[i]);
-      b->shots[i].h->inject_args(inj[i]);
+      b->shots[s0 + i].h->adj_args(adj[i]);
+      b->shots[s0 + i].h->inject_args(inj[i]);
       bs[i + 1] = bs[i] + (int)((adj[i].nupd + 255) / 256);
     }
     b->adj_blocks = bs[S];
@@ -529,10 +532,9 @@ extern "C" int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, con
   // copy tables: [save ckpt k] nseg, [load ckpt k] nseg, [save slot q] every + 1; 4 pieces per shot
   const int per = 4 * S;
   auto pieces = [&](biot_handle *h, char *state, bool to_state, CopyEntry *e) {
-    const size_t fb = (size_t)NFIELD * h->n * h->esz;
     char *live[4] = {(char *)h->d_fields, (char *)h->d_carry[0], (char *)h->d_carry[1], (char *)h->d_carry[2]};
-    const size_t bytes[4] = {fb, (size_t)NCARRY * h->slab[0] * h->csz, (size_t)NCARRY * h->slab[1] * h->csz,
-                             (size_t)NCARRY * h->slab[2] * h->csz};
+    const size_t bytes[4] = {(size_t)NFIELD * h->n * h->esz, (size_t)NCARRY * h->slab[0] * h->csz,
+                             (size_t)NCARRY * h->slab[1] * h->csz, (size_t)NCARRY * h->slab[2] * h->csz};
     for (int p = 0; p < 4; ++p) {
       char *st = state + h->state_off(p);
       e[p].src = to_state ? (const void *)live[p] : (const void *)st;
@@ -542,57 +544,50 @@ extern "C" int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, con
   };
   {
     std::vector<CopyEntry> tab((size_t)(2 * nseg + every + 1) * per);
-    for (int k = 0; k < nseg; ++k)
-      for (int i = 0; i < S; ++i) {
-        biot_handle *h = b->shots[i].h;
-        pieces(h, (char *)b->shots[i].ckpt[k], true, &tab[(size_t)k * per + 4 * i]);
-        pieces(h, (char *)b->shots[i].ckpt[k], false, &tab[(size_t)(nseg + k) * per + 4 * i]);
+    for (int i = 0; i < S; ++i) {
+      biot_handle *h = b->shots[s0 + i].h;
+      for (int k = 0; k < nseg; ++k) {
+        pieces(h, (char *)b->shots[s0 + i].ckpt[k], true, &tab[(size_t)k * per + 4 * i]);
+        pieces(h, (char *)b->shots[s0 + i].ckpt[k], false, &tab[(size_t)(nseg + k) * per + 4 * i]);
       }
-    for (int q = 0; q <= every; ++q)
-      for (int i = 0; i < S; ++i) {
-        biot_handle *h = b->shots[i].h;
-        pieces(h, (char *)h->d_slots + (size_t)q * h->state_bytes(), true,
-               &tab[(size_t)(2 * nseg + q) * per + 4 * i]);
-      }
+      for (int q = 0; q <= every; ++q)
+        pieces(h, (char *)h->d_slots + (size_t)q * h->state_bytes(), true, &tab[(size_t)(2 * nseg + q) * per + 4 * i]);
+    }
     BHIP(b, hipMalloc((void **)&b->d_copy, tab.size() * sizeof(CopyEntry)));
     BHIP(b, hipMemcpy(b->d_copy, tab.data(), tab.size() * sizeof(CopyEntry), hipMemcpyHostToDevice));
   }
   auto save_ckpt = [&](int k) { return launch_copy_batch(b->d_copy + (size_t)k * per, per, b->stream); };
   auto load_ckpt = [&](int k) { return launch_copy_batch(b->d_copy + (size_t)(nseg + k) * per, per, b->stream); };
   auto save_slot = [&](int q) { return launch_copy_batch(b->d_copy + (size_t)(2 * nseg + q) * per, per, b->stream); };
-
-  for (int i = 0; i < S; ++i) {
-    biot_handle *h = b->shots[i].h;
-    const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
-    BHIP(b, hipMemcpy(b->shots[i].d_obs, observed[i], (size_t)m * sizeof(float), hipMemcpyHostToDevice));
+  for (int i = s0; i < s1; ++i) {
+    int r = biot_reset(b->shots[i].h);
+    if (r) return hfail(b, b->shots[i].h, r, "reset");
   }
-  if ((rc = reset_all(b)) || (rc = set_counter(b, 0))) return rc;
+  if ((rc = set_counter(b, 0))) return rc;
 
-  BHIP(b, hipEventRecord(b->ev0, b->stream));
   // ---- forward with checkpoints
   for (int k = 0; k < nseg; ++k) {
     BHIP(b, save_ckpt(k));
     if ((rc = run_steps(b, std::min(every, nt - k * every), 0, true))) return rc;
   }
   // ---- FP64 misfit per shot; the residual is the adjoint source
-  BHIP(b, hipMemsetAsync(b->d_J, 0, sizeof(double), b->stream));
-  for (auto &s : b->shots) {
-    biot_handle *h = s.h;
+  for (int i = s0; i < s1; ++i) {
+    biot_handle *h = b->shots[i].h;
     const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
     const bool dbl = h->d_traces64 != nullptr;
-    BHIP(b, launch_misfit(dbl ? (const void *)h->d_traces64 : h->d_traces, dbl, s.d_obs, h->d_adjsrc, m,
-                          b->d_J, b->stream));
+    BHIP(b, launch_misfit(dbl ? (const void *)h->d_traces64 : h->d_traces, dbl, b->shots[i].d_obs,
+                          h->d_adjsrc, m, b->d_J, b->stream));
     BHIP(b, hipMemsetAsync(h->d_lam, 0, h->lam_bytes(), b->stream));
     BHIP(b, hipMemsetAsync(h->d_grad, 0, (size_t)NKC * h->n * sizeof(double), b->stream));
   }
-  // ---- segment recompute and adjoint sweep, all shots per launch
+  // ---- segment recompute and adjoint sweep, all shots of the group per launch
   const AdjBatch ab{b->d_adj, b->d_blocks, S, b->adj_blocks};
   for (int k = nseg - 1; k >= 0; --k) {
     const int n0 = k * every, K = std::min(every, nt - n0);
     BHIP(b, load_ckpt(k));
     BHIP(b, launch_set_counter(b->d_it, n0, b->stream));
+    const BatchArgs ba = batch_args(b);
     for (int q = 0; q < K; ++q) {
-      const BatchArgs ba = batch_args(b);
       BHIP(b, launch_source_batch(prec, b->d_src, S, b->d_it, b->stream));
       BHIP(b, save_slot(q));
       BHIP(b, launch_step_batch(prec, order, true, ba, b->shape, b->blocks, b->stream));
@@ -606,8 +601,8 @@ extern "C" int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, con
     }
   }
   // ---- chain rule per shot into the model-grid gradient
-  BHIP(b, hipMemsetAsync(b->d_gout, 0, (size_t)6 * N * sizeof(double), b->stream));
-  for (auto &s : b->shots) {
+  for (int i = s0; i < s1; ++i) {
+    auto &s = b->shots[i];
     biot_handle *h = s.h;
     ChainArgs c{};
     for (int k = 0; k < NKC; ++k) {
@@ -630,15 +625,96 @@ extern "C" int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, con
     c.Hinv = (double)(1.0f / b->cfg.dx);
     BHIP(b, launch_chain(c, b->stream));
   }
+  for (int i = s0; i < s1; ++i) b->shots[i].h->it = nt;
+  return BIOT_OK;
+}
+
+// releases the per-shot gradient buffers of shots [s0, s1) and the group tables
+void free_group(biot_batch *b, int s0, int s1)
+{
+  (void)hipStreamSynchronize(b->stream);
+  for (int i = s0; i < s1; ++i) {
+    auto &s = b->shots[i];
+    for (void *p : s.ckpt)
+      if (p) (void)hipFree(p);
+    s.ckpt.clear();
+    for (auto &p : s.d_par) dfree(p);
+    dfree(s.d_obs);
+    s.h->free_adjoint();
+  }
+  dfree(b->d_adj);
+  dfree(b->d_blocks);
+  dfree(b->d_inj);
+  dfree(b->d_copy);
+}
+
+}  // namespace
+
+extern "C" int biot_batch_gradient(biot_batch *b, int32_t nt, int32_t every, const float *const observed[],
+                                   double *J, double *const grad[6])
+{
+  if (!b) return BIOT_ERR_ARG;
+  if (!observed || !J || !grad) return bfail(b, BIOT_ERR_ARG, "observed, J and grad are required");
+  for (int q = 0; q < 6; ++q)
+    if (!grad[q]) return bfail(b, BIOT_ERR_ARG, "grad[%d] is null", q);
+  const int S = (int)b->shots.size();
+  if (S == 0) return bfail(b, BIOT_ERR_STATE, "no shots");
+  if (nt <= 0 || every <= 0) return bfail(b, BIOT_ERR_ARG, "nt and every must be positive");
+  for (int i = 0; i < S; ++i) {
+    if (!observed[i]) return bfail(b, BIOT_ERR_ARG, "observed[%d] is null", i);
+    if (b->shots[i].h->nt_max < nt) return bfail(b, BIOT_ERR_ARG, "shot %d records fewer than nt steps", i);
+    if (b->shots[i].h->src_mask & 0x3Fu)
+      return bfail(b, BIOT_ERR_ARG, "shot %d: the adjoint path supports sources on stresses and pressure only", i);
+  }
+  const int nseg = (nt + every - 1) / every;
+  const int64_t N = (int64_t)b->cfg.nx * b->cfg.ny * b->cfg.nz;
+  b->free_gradient();
+  BHIP(b, hipMalloc((void **)&b->d_gout, (size_t)6 * N * sizeof(double)));
+  BHIP(b, hipMalloc((void **)&b->d_J, sizeof(double)));
+  // shots go in groups that fit the free device memory (one group when they all fit)
+  size_t free_b = 0, total_b = 0;
+  BHIP(b, hipMemGetInfo(&free_b, &total_b));
+  const size_t budget = (size_t)((double)free_b * 0.9);
+  std::vector<std::pair<int, int>> groups;
+  for (int i = 0; i < S;) {
+    size_t used = 0;
+    int j = i;
+    while (j < S) {
+      const size_t need = shot_gradient_bytes(b->shots[j].h, nseg, every);
+      if (j > i && used + need > budget) break;
+      used += need;
+      ++j;
+    }
+    groups.emplace_back(i, j);
+    i = j;
+  }
+  BHIP(b, hipMemsetAsync(b->d_gout, 0, (size_t)6 * N * sizeof(double), b->stream));
+  BHIP(b, hipMemsetAsync(b->d_J, 0, sizeof(double), b->stream));
+  BHIP(b, hipEventRecord(b->ev0, b->stream));
+  int rc = BIOT_OK;
+  for (auto &g : groups) {
+    rc = gradient_group(b, g.first, g.second, nt, every, observed);
+    free_group(b, g.first, g.second);
+    if (rc) break;
+  }
+  b->built = false;
+  if (rc) return rc;
   BHIP(b, hipEventRecord(b->ev1, b->stream));
   BHIP(b, hipEventSynchronize(b->ev1));
   float ms = 0;
   BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_ms = ms;
+  b->groups = (int)groups.size();
   BHIP(b, hipMemcpy(J, b->d_J, sizeof(double), hipMemcpyDeviceToHost));
   for (int q = 0; q < 6; ++q)
     BHIP(b, hipMemcpy(grad[q], b->d_gout + (size_t)q * N, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
-  for (auto &s : b->shots) s.h->it = nt;
+  return BIOT_OK;
+}
+
+extern "C" int biot_batch_gradient_groups(biot_batch *b, int32_t *groups)
+{
+  if (!b || !groups) return BIOT_ERR_ARG;
+  *groups = b->groups;
   return BIOT_OK;
 }
 
