@@ -181,7 +181,7 @@ def adjoint_shot(repeats=2):
     return best
 
 
-def batch_shots(nt=1000, every=50):
+def batch_shots(nt=1000, every=50, graph_steps=50):
     """16 shots in cubic windows of 96..192 cells (seeded) cut from the 512 x 512 x 384
     perturbed medium, FP16 wavefields: the graph-captured batched forward (throughput over
     all windows) and the batched misfit gradient (checkpoints every 50 steps, FP64 sums,
@@ -196,9 +196,9 @@ def batch_shots(nt=1000, every=50):
     cells = sum(int(np.prod(sh["size"])) for sh in cfg["shots"])
     log(f"[bench] batch: {len(cfg['shots'])} shots, {cells / 1e6:.1f} M cells, set-up {time.time() - t0:.1f} s")
     launch = b.autotune(3)
-    b.forward(nt, graph_steps=50)
+    b.forward(nt, graph_steps=graph_steps)
     obs = [b.traces(i) for i in range(len(cfg["shots"]))]
-    b.forward(nt, graph_steps=50)
+    b.forward(nt, graph_steps=graph_steps)
     f_ms = b.last_ms()
     # observed data: the same shots with a slower, softer start model would need a second
     # model; the gradient's cost does not depend on the data, so the recorded traces scaled
@@ -214,9 +214,26 @@ def batch_shots(nt=1000, every=50):
                          f"perturbed medium, order 4, PML 20, {nt} steps, FP16 wavefields",
                 shots=len(cfg["shots"]), cells=cells, nt=nt,
                 forward_ms=round(f_ms, 1), forward_Mcell_updates_per_s=round(cells * nt / f_ms / 1e3, 1),
-                graph_steps=50, launch_bx_by_zchunk_blocks=list(launch),
+                graph_steps=graph_steps, launch_bx_by_zchunk_blocks=list(launch),
                 gradient_ms=round(g_ms, 1), gradient_ms_per_shot=round(g_ms / len(cfg["shots"]), 1),
                 checkpoint_every=every, gradient_groups=groups, misfit=J, finite=finite)
+
+
+def kernel_probe(steps):
+    """Each stencil variant at a fixed tile for `steps` steps of the 256^3 workload (for
+    hardware-counter passes): FP32 and FP16 one cell per lane (128x1 threads, 4 planes),
+    FP16 four cells per lane (64x4 threads, 2 planes)."""
+    from biot_amd import configs
+
+    cfg = configs.layered256(order=4, nt=steps)
+    out = {}
+    for prec, shape in (("fp32", (128, 1, 4, 1)), ("fp16", (128, 1, 4, 1)), ("fp16", (64, 4, 2, 4))):
+        s = configs.make_solver(cfg, precision=prec, nt=steps)
+        s.set_launch_shape(*shape)
+        v, t = s.profile_steps(steps)
+        out[f"{prec} xv{shape[3]}"] = {"velocity_ms": round(v, 4), "stress_ms": round(t, 4)}
+        s.close()
+    print(json.dumps(out))
 
 
 def copy_calibration(mib):
@@ -237,6 +254,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB")
+    ap.add_argument("--kernel-probe", type=int, default=0, metavar="STEPS")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
@@ -245,9 +263,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-graph-steps", type=int, default=50,
+                    help="steps per captured HIP graph in the batched forward (0: launch step by step)")
     args = ap.parse_args()
     if args.copy_calib:
         copy_calibration(args.copy_calib)
+        return
+    if args.kernel_probe:
+        kernel_probe(args.kernel_probe)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solver holds the whole grid on a single MI355X "
@@ -321,7 +344,7 @@ def main():
         log(f"[bench] adjoint: {out['adjoint']['ms_per_shot']:.0f} ms/shot")
     if not args.no_batch:
         log("[bench] batched shots ...")
-        out["batch"] = batch_shots()
+        out["batch"] = batch_shots(graph_steps=args.batch_graph_steps)
         log(f"[bench] batch: forward {out['batch']['forward_Mcell_updates_per_s']:.0f} Mcell/s, "
             f"gradient {out['batch']['gradient_ms']:.0f} ms")
     if not args.no_cpu:
