@@ -149,6 +149,21 @@ int biot_last_run_ms(biot_handle *h, double *ms);
  * stencil kernels on the handle's stream; returns their mean durations in ms. */
 int biot_profile_steps(biot_handle *h, int32_t nsteps, double *ms_velocity, double *ms_stress);
 
+/* ---- output ---- */
+/* The reference's text format (wfile, outputfunc.cpp:35-50): nr rows of nc values, each value
+ * printed "%e ", each row followed by a newline and an empty line.  data is row-major. */
+int biot_write_text_2d(const char *path, const float *data, int32_t nr, int32_t nc);
+/* The y = ny/2 slice of a (nz, ny, nx) volume in the same format (wfile1d / wfile3d,
+ * outputfunc.cpp:97-151): nz rows of nx values. */
+int biot_write_text_xz(const char *path, const float *vol, int32_t nz, int32_t ny, int32_t nx);
+/* count float32 values, raw little-endian (the binary option beside the text files). */
+int biot_write_binary(const char *path, const float *data, int64_t count);
+/* nsteps from the current step, like biot_run, with the recorded traces copied into out
+ * (trace layout, ncomp*nt_max*nrec floats) on a side stream chunk steps at a time while
+ * the loop continues; returns when every copy has landed.  Replaces the reference's
+ * per-step full-volume copy to the host (kernel.cu:2197-2211). */
+int biot_run_stream_traces(biot_handle *h, int32_t nsteps, int32_t chunk, float *out);
+
 /* ---- inversion parameterization and the adjoint (no counterpart in the reference) ---- */
 /* Per-cell inversion parameters (nx*ny*nz doubles each).  lambda (drained Lame), mu, alpha,
  * M, rho, rho_f are the inverted quantities; C1 = eta/kappa and tphi = C2/rho_f

@@ -134,6 +134,10 @@ def lib():
         "biot_get_field_scales": ([H, FP], C.c_int),
         "biot_device_bytes": ([H, C.POINTER(i64)], C.c_int),
         "biot_set_params": ([H, C.POINTER(Params)], C.c_int),
+        "biot_write_text_2d": ([C.c_char_p, FP, i32, i32], C.c_int),
+        "biot_write_text_xz": ([C.c_char_p, FP, i32, i32, i32], C.c_int),
+        "biot_write_binary": ([C.c_char_p, FP, i64], C.c_int),
+        "biot_run_stream_traces": ([H, i32, i32, FP], C.c_int),
         "biot_forward_checkpointed": ([H, i32, i32], C.c_int),
         "biot_misfit_l2": ([H, FP, FP, _DP], C.c_int),
         "biot_adjoint": ([H, FP, C.POINTER(_DP), C.POINTER(_DP)], C.c_int),
@@ -176,7 +180,8 @@ def exported_symbols():
             "biot_set_launch_shape", "biot_get_launch_shape", "biot_autotune",
             "biot_read_traces", "biot_read_traces_f64", "biot_read_field",
             "biot_read_field_f64", "biot_write_field_f64", "biot_snapshot_xz",
-            "biot_get_field_scales", "biot_device_bytes", "biot_set_params",
+            "biot_get_field_scales", "biot_device_bytes", "biot_write_text_2d",
+            "biot_write_text_xz", "biot_write_binary", "biot_run_stream_traces", "biot_set_params",
             "biot_forward_checkpointed", "biot_misfit_l2", "biot_adjoint",
             "biot_params_coefficients", "biot_params_chain", "biot_batch_create",
             "biot_batch_destroy", "biot_batch_last_error", "biot_batch_add_shot",
@@ -281,3 +286,30 @@ def params_chain(nx, ny, nz, dx, dt, params, grad_coef, Cv=1e8, Cp=1.0):
     if rc != BIOT_OK:
         raise BiotError(rc, "biot_params_chain")
     return out
+
+
+def write_text_2d(path, data):
+    """The reference's text format (outputfunc.cpp:35-50) for a 2-D array (rows x columns)."""
+    a = np.ascontiguousarray(np.asarray(data, np.float32))
+    if a.ndim != 2:
+        raise ValueError("2-D array expected")
+    rc = lib().biot_write_text_2d(os.fsencode(path), fptr(a), a.shape[0], a.shape[1])
+    if rc != BIOT_OK:
+        raise BiotError(rc, f"biot_write_text_2d({path})")
+
+
+def write_text_xz(path, volume):
+    """The y = ny/2 slice of a (nz, ny, nx) volume in the reference's text format."""
+    a = np.ascontiguousarray(np.asarray(volume, np.float32))
+    if a.ndim != 3:
+        raise ValueError("(nz, ny, nx) volume expected")
+    rc = lib().biot_write_text_xz(os.fsencode(path), fptr(a), *a.shape)
+    if rc != BIOT_OK:
+        raise BiotError(rc, f"biot_write_text_xz({path})")
+
+
+def write_binary(path, data):
+    a = np.ascontiguousarray(np.asarray(data, np.float32)).reshape(-1)
+    rc = lib().biot_write_binary(os.fsencode(path), fptr(a), a.size)
+    if rc != BIOT_OK:
+        raise BiotError(rc, f"biot_write_binary({path})")

@@ -187,6 +187,25 @@ class Solver:
         self._check(lib().biot_device_bytes(self._h, C.byref(b)), "biot_device_bytes")
         return b.value
 
+    def run_stream_traces(self, nsteps, chunk=100):
+        """nsteps like run(), the traces copied to the host on a side stream chunk steps at a
+        time while the loop continues; returns them, shape (ncomp, nt_max, nrec)."""
+        out = np.zeros((self.ncomp, self.nt_max, self.nrec), np.float32)
+        self._check(lib().biot_run_stream_traces(self._h, int(nsteps), int(chunk), fptr(out)),
+                    "biot_run_stream_traces")
+        return out
+
+    def write_traces_text(self, path, comp=0, traces=None):
+        """One recorded component in the reference's trace-file layout (sisx*.dat,
+        kernel.cu:2557-2571): one row per receiver, nt_max columns."""
+        tr = self.traces() if traces is None else traces
+        _lib.write_text_2d(path, tr[comp].T)
+
+    def write_snapshot_text(self, path, name):
+        """The y = ny/2 xz slice of a field in the reference's snapshot format
+        (txxpro*.dat, kernel.cu:2247-2385)."""
+        _lib.write_text_2d(path, self.snapshot_xz(name, self.ny // 2))
+
     def traces_f64(self):
         """Receiver traces in double (FP64 handles only), shape (ncomp, nt_max, nrec)."""
         out = np.zeros((self.ncomp, self.nt_max, self.nrec), np.float64)
