@@ -116,6 +116,17 @@ int biot_get_coefficients(biot_handle *h, float *const coef[BIOT_NCOEF]);
 int biot_get_pml_profile(biot_handle *h, int32_t axis, float *d, float *d2, float *e,
                          float *e2);
 
+/* biot_set_medium with the moduli, averages and coefficients built on the device (the same
+ * bits as the host builder); for large grids, where the host set-up dominates. */
+int biot_set_medium_device(biot_handle *h, const biot_medium *m);
+/* Host only: reference arrays -> the 22 kernel coefficient arrays before storage scaling
+ * (1/dx as float, kernel.cu:158, and the 0.01 of kernel.cu:484-490 folded; n cells). */
+int biot_kernel_coefficients(float dx, int64_t n, const float *const coef[BIOT_NCOEF],
+                             double *const K[22]);
+/* The stored kernel coefficient array k (enum biot_kcoef) as double, times its storage
+ * scale for 16-bit storage. */
+int biot_read_kernel_coefficients(biot_handle *h, int32_t k, double *out);
+
 /* Host-only forms of the two set-up stages above (no device is touched): the 23
  * coefficient arrays of main.cpp:431-494 from per-cell moduli (nx*ny*nz floats each), and
  * the PML profile of one axis of length n. */

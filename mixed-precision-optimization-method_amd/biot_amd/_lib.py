@@ -134,6 +134,9 @@ def lib():
         "biot_get_field_scales": ([H, FP], C.c_int),
         "biot_device_bytes": ([H, C.POINTER(i64)], C.c_int),
         "biot_set_params": ([H, C.POINTER(Params)], C.c_int),
+        "biot_set_medium_device": ([H, C.POINTER(Medium)], C.c_int),
+        "biot_kernel_coefficients": ([C.c_float, i64, C.POINTER(FP), C.POINTER(_DP)], C.c_int),
+        "biot_read_kernel_coefficients": ([H, i32, _DP], C.c_int),
         "biot_write_text_2d": ([C.c_char_p, FP, i32, i32], C.c_int),
         "biot_write_text_xz": ([C.c_char_p, FP, i32, i32, i32], C.c_int),
         "biot_write_binary": ([C.c_char_p, FP, i64], C.c_int),
@@ -182,6 +185,7 @@ def exported_symbols():
             "biot_read_field_f64", "biot_write_field_f64", "biot_snapshot_xz",
             "biot_get_field_scales", "biot_device_bytes", "biot_write_text_2d",
             "biot_write_text_xz", "biot_write_binary", "biot_run_stream_traces", "biot_set_params",
+            "biot_set_medium_device", "biot_kernel_coefficients", "biot_read_kernel_coefficients",
             "biot_forward_checkpointed", "biot_misfit_l2", "biot_adjoint",
             "biot_params_coefficients", "biot_params_chain", "biot_batch_create",
             "biot_batch_destroy", "biot_batch_last_error", "biot_batch_add_shot",
@@ -313,3 +317,15 @@ def write_binary(path, data):
     rc = lib().biot_write_binary(os.fsencode(path), fptr(a), a.size)
     if rc != BIOT_OK:
         raise BiotError(rc, f"biot_write_binary({path})")
+
+
+def kernel_coefficients(dx, coefs):
+    """Host-only: the 22 unscaled kernel coefficient arrays (double) of the 23 reference arrays."""
+    arrs = [np.ascontiguousarray(np.asarray(c, np.float32).reshape(-1)) for c in coefs]
+    n = arrs[0].size
+    out = [np.zeros(n, np.float64) for _ in range(NKCOEF)]
+    rc = lib().biot_kernel_coefficients(C.c_float(dx), n, (C.POINTER(C.c_float) * NCOEF)(*[fptr(a) for a in arrs]),
+                                        (_DP * NKCOEF)(*[dptr(a) for a in out]))
+    if rc != BIOT_OK:
+        raise BiotError(rc, "biot_kernel_coefficients")
+    return out

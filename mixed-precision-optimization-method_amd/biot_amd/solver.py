@@ -76,6 +76,22 @@ class Solver:
         m = Medium(**{k: dptr(arrs[k]) for k in MEDIUM_KEYS}, tao=tao, porousm=porousm)
         self._check(lib().biot_set_medium(self._h, C.byref(m)), "biot_set_medium")
 
+    def set_medium_device(self, medium, tao=3.0, porousm=8.0):
+        """set_medium with the moduli, averages and coefficients built on the GPU."""
+        arrs = {}
+        for k in MEDIUM_KEYS:
+            v = np.asarray(medium[k], dtype=np.float64)
+            arrs[k] = np.ascontiguousarray(np.broadcast_to(v, self.shape)).reshape(-1)
+        m = Medium(**{k: dptr(arrs[k]) for k in MEDIUM_KEYS}, tao=tao, porousm=porousm)
+        self._check(lib().biot_set_medium_device(self._h, C.byref(m)), "biot_set_medium_device")
+
+    def kernel_coefficient(self, k):
+        """Stored kernel coefficient array k (enum biot_kcoef order) as double."""
+        out = np.zeros(self.n, np.float64)
+        self._check(lib().biot_read_kernel_coefficients(self._h, int(k), dptr(out)),
+                    "biot_read_kernel_coefficients")
+        return out.reshape(self.shape)
+
     def set_coefficients(self, coefs):
         arrs = [np.ascontiguousarray(np.asarray(c, np.float32).reshape(-1)) for c in coefs]
         ptrs = (C.POINTER(C.c_float) * NCOEF)(*[fptr(a) for a in arrs])

@@ -240,9 +240,8 @@ extern "C" int biot_destroy(biot_handle *h)
 // ------------------------------------------------------------------------------------
 // medium / coefficients
 // ------------------------------------------------------------------------------------
-extern "C" int biot_set_medium(biot_handle *h, const biot_medium *m)
+static int check_medium(biot_handle *h, const biot_medium *m)
 {
-  if (!h) return BIOT_ERR_ARG;
   if (!m || !m->Ks || !m->Kf || !m->Kb || !m->mu || !m->phi || !m->rhos || !m->rhof || !m->eta ||
       !m->perm)
     return fail(h, BIOT_ERR_ARG, "biot_set_medium: every per-cell array must be given");
@@ -254,6 +253,14 @@ extern "C" int biot_set_medium(biot_handle *h, const biot_medium *m)
                    !(m->eta[c] >= 0) || !(m->mu[c] >= 0) || !(m->Kb[c] >= 0)))
       return fail(h, BIOT_ERR_ARG, "cell %lld: invalid porous-medium parameters", (long long)c);
   }
+  return BIOT_OK;
+}
+
+extern "C" int biot_set_medium(biot_handle *h, const biot_medium *m)
+{
+  if (!h) return BIOT_ERR_ARG;
+  int rc0 = check_medium(h, m);
+  if (rc0) return rc0;
   h->forget_model();
   Moduli md;
   Averages av;
@@ -261,6 +268,101 @@ extern "C" int biot_set_medium(biot_handle *h, const biot_medium *m)
   build_averages(h->g.nx, h->g.ny, h->g.nz, md, av);
   build_coefficients(h->g.nx, h->g.ny, h->g.nz, h->g.dt, h->g.Cv, h->g.Cp, md, av, h->coef_ref);
   return h->upload_coefficients();
+}
+
+// The same set-up on the device: medium arrays uploaded once, moduli, averages and the 23
+// coefficient arrays built by kernels (bit-identical to the host builder), converted to the
+// kernel arrays there; the reference arrays are copied back for biot_get_coefficients.
+extern "C" int biot_set_medium_device(biot_handle *h, const biot_medium *m)
+{
+  if (!h) return BIOT_ERR_ARG;
+  int rc = check_medium(h, m);
+  if (rc) return rc;
+  h->forget_model();
+  const int64_t n = h->n;
+  double *d_med = nullptr;
+  float *d_mod = nullptr, *d_ref = nullptr;
+  if (hipMalloc((void **)&d_med, (size_t)9 * n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&d_mod, (size_t)9 * n * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&d_ref, (size_t)BIOT_NCOEF * n * sizeof(float)) != hipSuccess) {
+    for (void *p : {(void *)d_med, (void *)d_mod, (void *)d_ref})
+      if (p) (void)hipFree(p);
+    return fail(h, BIOT_ERR_NOMEM, "set-up buffers do not fit");
+  }
+  const double *src[9] = {m->Ks, m->Kf, m->Kb, m->mu, m->phi, m->rhos, m->rhof, m->eta, m->perm};
+  for (int q = 0; q < 9 && !rc; ++q)
+    if (hipMemcpy(d_med + (size_t)q * n, src[q], (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      rc = BIOT_ERR_HIP;
+  MediumArgs ma;
+  std::memset(&ma, 0, sizeof ma);
+  const double **mp[9] = {&ma.Ks, &ma.Kf, &ma.Kb, &ma.mu, &ma.phi, &ma.rhos, &ma.rhof, &ma.eta, &ma.perm};
+  for (int q = 0; q < 9; ++q) *mp[q] = d_med + (size_t)q * n;
+  ma.tao = m->tao;
+  ma.porousm = m->porousm;
+  for (int q = 0; q < 9; ++q) ma.out[q] = d_mod + (size_t)q * n;
+  CoefRefArgs ca;
+  std::memset(&ca, 0, sizeof ca);
+  for (int q = 0; q < 9; ++q) ca.mod[q] = d_mod + (size_t)q * n;
+  for (int q = 0; q < BIOT_NCOEF; ++q) ca.coef[q] = d_ref + (size_t)q * n;
+  ca.nx = h->g.nx;
+  ca.ny = h->g.ny;
+  ca.nz = h->g.nz;
+  ca.DT = h->g.dt;
+  ca.Cv = h->g.Cv;
+  ca.Cp = h->g.Cp;
+  if (!rc && (launch_moduli(ma, n, h->stream) != hipSuccess || launch_coef_ref(ca, h->stream) != hipSuccess ||
+              hipStreamSynchronize(h->stream) != hipSuccess))
+    rc = BIOT_ERR_HIP;
+  for (int q = 0; q < BIOT_NCOEF && !rc; ++q) {
+    h->coef_ref[q].resize((size_t)n);
+    if (hipMemcpy(h->coef_ref[q].data(), d_ref + (size_t)q * n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      rc = BIOT_ERR_HIP;
+  }
+  if (rc) {
+    for (void *p : {(void *)d_med, (void *)d_mod, (void *)d_ref}) (void)hipFree(p);
+    return fail(h, rc, "device set-up failed");
+  }
+  (void)hipFree(d_med);
+  (void)hipFree(d_mod);
+  rc = h->upload_from_ref(d_ref);
+  (void)hipFree(d_ref);
+  return rc;
+}
+
+extern "C" int biot_kernel_coefficients(float dx, int64_t n, const float *const coef[BIOT_NCOEF],
+                                        double *const K[22])
+{
+  if (!(dx > 0) || n <= 0 || !coef || !K) return BIOT_ERR_ARG;
+  std::vector<float> ref[BIOT_NCOEF];
+  for (int q = 0; q < BIOT_NCOEF; ++q) {
+    if (!coef[q] && q != BIOT_W3Z) return BIOT_ERR_ARG;
+    if (coef[q]) ref[q].assign(coef[q], coef[q] + n);
+    else ref[q].assign((size_t)n, 0.0f);
+  }
+  for (int k = 0; k < NKC; ++k)
+    if (!K[k]) return BIOT_ERR_ARG;
+  std::vector<double> Kv[NKC];
+  host_kernel_coefs((double)(1.0f / dx), ref, Kv);
+  for (int k = 0; k < NKC; ++k) std::memcpy(K[k], Kv[k].data(), (size_t)n * sizeof(double));
+  return BIOT_OK;
+}
+
+extern "C" int biot_read_kernel_coefficients(biot_handle *h, int32_t k, double *out)
+{
+  if (!h || !out) return BIOT_ERR_ARG;
+  if (k < 0 || k >= NKC) return fail(h, BIOT_ERR_ARG, "kernel coefficient %d out of range", k);
+  if (!h->have_coef) return fail(h, BIOT_ERR_STATE, "no coefficients set");
+  std::vector<unsigned char> buf((size_t)h->n * h->esz);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(buf.data(), (const char *)h->d_coef + (size_t)k * h->n * h->esz, buf.size(),
+                      hipMemcpyDeviceToHost));
+  for (int64_t c = 0; c < h->n; ++c) {
+    if (h->prec == BIOT_FP64) out[c] = ((const double *)buf.data())[c];
+    else if (h->prec == BIOT_FP32) out[c] = ((const float *)buf.data())[c];
+    else out[c] = (double)(float)((const _Float16 *)buf.data())[c] * (double)h->ks[k];
+  }
+  return BIOT_OK;
 }
 
 extern "C" int biot_build_coefficients(int32_t nx, int32_t ny, int32_t nz, float dt, float Cv,

@@ -101,6 +101,33 @@ struct RecordArgs {
 };
 hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s);
 
+// GPU-side set-up (biot_build.hip), bit-identical to the host builder.
+struct MediumArgs {
+  const double *Ks, *Kf, *Kb, *mu, *phi, *rhos, *rhof, *eta, *perm;
+  double tao, porousm;
+  float *out[9];            // M HH H2u C C1 C2 rho rhof mu
+};
+struct CoefRefArgs {
+  const float *mod[9];      // as MediumArgs::out
+  float *coef[23];          // the reference arrays (enum biot_coef order)
+  int nx, ny, nz;
+  float DT, Cv, Cp;
+};
+struct RefArgs {
+  const float *ref[23];     // reference arrays on the device
+  int64_t n;
+  double H;                 // 1/dx as the kernels hold it
+  uint32_t *stats_u32;      // 23 maxima of |ref| (float bits)
+  unsigned long long *stats_u64;  // forced q response (double bits)
+  void *kout;               // 22 kernel arrays of the storage type
+  int scaled;
+  double ratio[NKC], inv_ks[NKC];
+};
+hipError_t launch_moduli(const MediumArgs &m, int64_t n, hipStream_t s);
+hipError_t launch_coef_ref(const CoefRefArgs &a, hipStream_t s);
+hipError_t launch_ref_stats(const RefArgs &a, hipStream_t s);
+hipError_t launch_ref_to_kernel(int prec, const RefArgs &a, hipStream_t s);
+
 // Batched launches over many shots (one grid for all; device-resident argument tables).
 struct BatchArgs {
   const StepArgs *shots;    // device, one per shot (gx, gy, gz, zchunk set for the shape)

@@ -345,7 +345,19 @@ double amax(const std::vector<double> &v)
 }
 }  // namespace
 
-// Builds the 22 kernel coefficient arrays from the reference arrays and uploads them.
+// host form of the reference -> kernel coefficient conversion (before storage scaling)
+void host_kernel_coefs(double Hinv, const std::vector<float> (&ref)[BIOT_NCOEF], std::vector<double> (&K)[NKC])
+{
+  for (auto &v : K) v.assign(ref[0].size(), 0.0);
+  KernCtx cx{ref, Hinv, K};
+  parallel_for((int64_t)ref[0].size(), kern_body, &cx);
+}
+namespace {
+}  // namespace
+
+// Builds the 22 kernel coefficient arrays from the reference arrays and uploads them: the
+// reference arrays go to the device once and the conversion (1/dx, 0.01, storage scales,
+// storage type) runs there (upload_from_ref).
 int biot_handle::upload_coefficients()
 {
   if (coef_ref[0].empty()) {
@@ -353,88 +365,73 @@ int biot_handle::upload_coefficients()
     err = "no coefficients to upload";
     return BIOT_ERR_STATE;
   }
-  std::vector<double> K[NKC];
-  for (auto &v : K) v.assign((size_t)n, 0.0);
-  // 1/dx as the float the reference's kernels hold (`float H`, kernel.cu:158)
-  KernCtx cx{coef_ref, (double)(1.0f / g.dx), K};
-  parallel_for(n, kern_body, &cx);
-  return upload_kernel_coefs(K);
+  float *d_ref = nullptr;
+  if (hipMalloc((void **)&d_ref, (size_t)BIOT_NCOEF * n * sizeof(float)) != hipSuccess) {
+    err = "reference coefficient buffer does not fit";
+    return BIOT_ERR_NOMEM;
+  }
+  int rc = BIOT_OK;
+  for (int q = 0; q < BIOT_NCOEF && !rc; ++q)
+    if (hipMemcpy(d_ref + (size_t)q * n, coef_ref[q].data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      rc = BIOT_ERR_HIP;
+  if (rc) err = "reference coefficient upload failed";
+  else rc = upload_from_ref(d_ref);
+  (void)hipFree(d_ref);
+  return rc;
 }
 
-// Scales (16-bit modes) and uploads kernel coefficient arrays.
-// FP16 storage: each field group is stored divided by a scale (stresses/pressure by the
-// peak source amplitude A; velocities by A*sqrt(max U1 / max S2), which makes the
+// 16-bit storage scales.  Each field group is stored divided by a scale (stresses/pressure
+// by the peak source amplitude A; velocities by A*sqrt(max U1 / max S2), which makes the
 // velocity and stress update coefficients both equal to the CFL number c*dt/dx; the
 // relative fluid velocity by its forced response A * max((|W2|+|W3|)/(1-W1)) * 0.2, the
 // 0.2 being k*dx of a wave sampled at ~30 points per wavelength); every coefficient array
-// is stored divided by its largest magnitude so it lies in [-1, 1].
-int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
+// is multiplied by the ratio of the scales it connects (kfac) and stored divided by its
+// largest magnitude (ks) so it lies in [-1, 1].  amaxK: per-array max |K| before scaling.
+void biot_handle::set_scales(const double amaxK[NKC], double resp)
 {
-  std::vector<double> K[NKC];
-  for (int k = 0; k < NKC; ++k) K[k] = K_in[k];
-
-  float old_scale[NFIELD];
-  std::memcpy(old_scale, fscale, sizeof old_scale);
   for (int f = 0; f < NFIELD; ++f) fscale[f] = 1.0f;
   for (int k = 0; k < NKC; ++k) ks[k] = 1.0f, kfac[k] = 1.0;
-  if (half()) {
-    double A = 0;
-    for (float w : wavelet) A = std::max(A, (double)std::fabs(w));
-    if (!(A > 0)) A = 1.0;
-    // 16x headroom: the stored source amplitude is 16, so the far field stays clear of the
-    // binary16 subnormal range; peaks at the source cell stay below ~1/16 of the largest
-    // binary16 (measured < 256 A on the layered and perturbed media)
-    A = std::ldexp(A, -4);
-    const double u1 = std::max({amax(K[K_U1X]), amax(K[K_U1Y]), amax(K[K_U1Z])});
-    const double s2 = amax(K[K_S2]);
-    const double sig_t = A, sig_p = A;
-    const double sig_v = (u1 > 0 && s2 > 0) ? A * std::sqrt(u1 / s2) : A;
-    double resp = 0;
-    for (int d = 0; d < 3; ++d) {
-      const auto &w1 = K[K_W1X + d], &w2 = K[K_W2X + d], &w3 = K[d == 2 ? K_W3Y : K_W3X + d];
-      for (int64_t c = 0; c < n; ++c) {
-        const double den = 1.0 - w1[c];
-        if (den > 0) resp = std::max(resp, (std::fabs(w2[c]) + std::fabs(w3[c])) / den);
-      }
-    }
-    const double sig_q = resp > 0 ? A * resp * 0.2 : sig_v;
-    const double sv = sig_v, sq = sig_q, stau = sig_t, sp = sig_p;
-    auto scale_arr = [&](int k, double f) {
-      for (auto &x : K[k]) x *= f;
-      kfac[k] *= f;
-    };
-    for (int d = 0; d < 3; ++d) {
-      scale_arr(K_U1X + d, stau / sv);
-      scale_arr(K_U2X + d, sq / sv);
-      scale_arr(K_W2X + d, stau / sq);
-    }
-    scale_arr(K_W3X, sp / sq);
-    scale_arr(K_W3Y, sp / sq);
-    scale_arr(K_P1, sv / sp);
-    scale_arr(K_P2, sq / sp);
-    scale_arr(K_S1, sv / stau);
-    scale_arr(K_S2, sv / stau);
-    scale_arr(K_S3, sq / stau);
-    scale_arr(K_SXY, sv / stau);
-    scale_arr(K_SXZ, sv / stau);
-    scale_arr(K_SYZ, sv / stau);
-    for (int f = 0; f < 3; ++f) fscale[f] = (float)sv;
-    for (int f = 3; f < 6; ++f) fscale[f] = (float)sq;
-    for (int f = 6; f < 12; ++f) fscale[f] = (float)stau;
-    fscale[12] = (float)sp;
-    for (int k = 0; k < NKC; ++k) {
-      const double m = amax(K[k]);
-      if (m > 0) {
-        ks[k] = (float)m;
-        for (auto &x : K[k]) x *= 1.0 / (double)ks[k];  // the kernels multiply ks back
-      }
-    }
+  if (!half()) return;
+  double A = 0;
+  for (float w : wavelet) A = std::max(A, (double)std::fabs(w));
+  if (!(A > 0)) A = 1.0;
+  // 16x headroom: the stored source amplitude is 16, so the far field stays clear of the
+  // binary16 subnormal range; peaks at the source cell stay below ~1/16 of the largest
+  // binary16 (measured < 256 A on the layered and perturbed media)
+  A = std::ldexp(A, -4);
+  const double u1 = std::max({amaxK[K_U1X], amaxK[K_U1Y], amaxK[K_U1Z]});
+  const double s2 = amaxK[K_S2];
+  const double sig_t = A, sig_p = A;
+  const double sig_v = (u1 > 0 && s2 > 0) ? A * std::sqrt(u1 / s2) : A;
+  const double sig_q = resp > 0 ? A * resp * 0.2 : sig_v;
+  const double sv = sig_v, sq = sig_q, stau = sig_t, sp = sig_p;
+  for (int d = 0; d < 3; ++d) {
+    kfac[K_U1X + d] = stau / sv;
+    kfac[K_U2X + d] = sq / sv;
+    kfac[K_W2X + d] = stau / sq;
   }
+  kfac[K_W3X] = kfac[K_W3Y] = sp / sq;
+  kfac[K_P1] = sv / sp;
+  kfac[K_P2] = sq / sp;
+  kfac[K_S1] = kfac[K_S2] = kfac[K_SXY] = kfac[K_SXZ] = kfac[K_SYZ] = sv / stau;
+  kfac[K_S3] = sq / stau;
+  for (int f = 0; f < 3; ++f) fscale[f] = (float)sv;
+  for (int f = 3; f < 6; ++f) fscale[f] = (float)sq;
+  for (int f = 6; f < 12; ++f) fscale[f] = (float)stau;
+  fscale[12] = (float)sp;
+  for (int k = 0; k < NKC; ++k) {
+    const double m = amaxK[k] * kfac[k];  // the largest scaled magnitude (rounding is monotonic)
+    if (m > 0) ks[k] = (float)m;
+  }
+}
 
-  // Stored wavefields and C-PML carries are expressed in the storage scales: when a scale
-  // changes (e.g. a new source amplitude), re-express what the handle already holds.
-  // Carries 0..3 of each axis are differences of stresses/pressure, 4..7 of solid
-  // velocities except carry 5 (relative fluid velocity).
+// Stored wavefields and C-PML carries are expressed in the storage scales: when a scale
+// changes (e.g. a new source amplitude), re-express what the handle already holds.
+// Carries 0..3 of each axis are differences of stresses/pressure, 4..7 of solid
+// velocities except carry 5 (relative fluid velocity).
+int biot_handle::rescale_state(const float old_scale[NFIELD])
+{
   auto rescale = [&](int p_, void *p, int8_t *lo, int64_t cnt, double f) -> bool {
     return f == 1.0 || launch_rescale(p_, p, lo, cnt, f, stream) == hipSuccess;
   };
@@ -454,6 +451,100 @@ int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
   if (!ok || hipStreamSynchronize(stream) != hipSuccess) {
     err = "rescaling the stored wavefields failed";
     return BIOT_ERR_HIP;
+  }
+  return BIOT_OK;
+}
+
+// Device-side conversion of the 23 reference arrays (d_ref, contiguous) into the kernel
+// arrays: statistics for the scales, then K = R (x 1/dx, x 0.01) x ratio / ks per cell.
+int biot_handle::upload_from_ref(const float *d_ref)
+{
+  void *stats = nullptr;
+  if (hipMalloc(&stats, 256) != hipSuccess) {
+    err = "statistics buffer";
+    return BIOT_ERR_NOMEM;
+  }
+  RefArgs ra;
+  std::memset(&ra, 0, sizeof ra);
+  for (int q = 0; q < BIOT_NCOEF; ++q) ra.ref[q] = d_ref + (size_t)q * n;
+  ra.n = n;
+  ra.H = (double)(1.0f / g.dx);  // 1/dx as the float the reference's kernels hold (kernel.cu:158)
+  ra.stats_u32 = (uint32_t *)stats;
+  ra.stats_u64 = (unsigned long long *)((char *)stats + 128);
+  uint32_t st32[BIOT_NCOEF];
+  unsigned long long st64 = 0;
+  int rc = BIOT_OK;
+  if (hipMemsetAsync(stats, 0, 256, stream) != hipSuccess || launch_ref_stats(ra, stream) != hipSuccess ||
+      hipMemcpyAsync(st32, stats, sizeof st32, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipMemcpyAsync(&st64, (char *)stats + 128, sizeof st64, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    rc = BIOT_ERR_HIP;
+  (void)hipFree(stats);
+  if (rc) {
+    err = "coefficient statistics failed";
+    return rc;
+  }
+  // per-array max |K| from max |R| (every conversion is a monotonic rounding)
+  auto amaxR = [&](int q) { return (double)__builtin_bit_cast(float, st32[q]); };
+  double amaxK[NKC];
+  const double H = ra.H;
+  for (int d = 0; d < 3; ++d) {
+    amaxK[K_U1X + d] = amaxR(BIOT_U1X + d) * H;
+    amaxK[K_U2X + d] = amaxR(BIOT_U2X + d);
+    amaxK[K_W1X + d] = amaxR(BIOT_W1X + d);
+    amaxK[K_W2X + d] = amaxR(BIOT_W2X + d) * H;
+  }
+  amaxK[K_W3X] = amaxR(BIOT_W3X) * H;
+  amaxK[K_W3Y] = amaxR(BIOT_W3Y) * H;
+  const int sref[8] = {BIOT_PP1, BIOT_PP2, BIOT_SP1, BIOT_SP2, BIOT_SP3, BIOT_SPXY, BIOT_SPXZ, BIOT_SPYZ};
+  for (int q = 0; q < 8; ++q) amaxK[K_P1 + q] = 0.01 * amaxR(sref[q]) * H;
+  const double resp = __builtin_bit_cast(double, st64);
+  float old_scale[NFIELD];
+  std::memcpy(old_scale, fscale, sizeof old_scale);
+  set_scales(amaxK, resp);
+  if ((rc = rescale_state(old_scale))) return rc;
+  ra.kout = d_coef;
+  ra.scaled = half() ? 1 : 0;
+  for (int k = 0; k < NKC; ++k) {
+    ra.ratio[k] = kfac[k];
+    ra.inv_ks[k] = 1.0 / (double)ks[k];
+  }
+  if (launch_ref_to_kernel(prec, ra, stream) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) {
+    err = "coefficient conversion failed";
+    return BIOT_ERR_HIP;
+  }
+  have_coef = true;
+  return BIOT_OK;
+}
+
+// Host-side form for coefficient sets built in double from inversion parameters.
+int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
+{
+  double amaxK[NKC];
+  for (int k = 0; k < NKC; ++k) amaxK[k] = amax(K_in[k]);
+  double resp = 0;
+  for (int d = 0; d < 3 && half(); ++d) {
+    const auto &w1 = K_in[K_W1X + d], &w2 = K_in[K_W2X + d], &w3 = K_in[d == 2 ? K_W3Y : K_W3X + d];
+    for (int64_t c = 0; c < n; ++c) {
+      const double den = 1.0 - w1[c];
+      if (den > 0) resp = std::max(resp, (std::fabs(w2[c]) + std::fabs(w3[c])) / den);
+    }
+  }
+  float old_scale[NFIELD];
+  std::memcpy(old_scale, fscale, sizeof old_scale);
+  set_scales(amaxK, resp);
+  int rc = rescale_state(old_scale);
+  if (rc) return rc;
+  std::vector<double> K[NKC];
+  for (int k = 0; k < NKC; ++k) {
+    K[k] = K_in[k];
+    if (half()) {
+      const double inv = 1.0 / (double)ks[k];
+      for (auto &x : K[k]) {
+        x *= kfac[k];
+        x *= inv;  // the kernels multiply ks back
+      }
+    }
   }
 
   std::vector<unsigned char> buf((size_t)n * esz);

@@ -66,6 +66,9 @@ struct biot_handle {
   int upload_profiles();
   int upload_coefficients();
   int upload_kernel_coefs(const std::vector<double> (&K)[biot::NKC]);
+  int upload_from_ref(const float *d_ref);      // 23 reference arrays on the device
+  void set_scales(const double amaxK[biot::NKC], double resp);
+  int rescale_state(const float old_scale[biot::NFIELD]);
 
   // inversion parameters (biot_set_params) and the adjoint's buffers
   std::vector<double> par[8];                // lambda mu alpha M rho rhof C1 tphi
@@ -95,3 +98,6 @@ struct biot_handle {
 };
 
 biot::LaunchShape default_shape(int prec, int order);
+// reference arrays -> unscaled kernel coefficients on the host (1/dx and 0.01 folded)
+void host_kernel_coefs(double Hinv, const std::vector<float> (&ref)[BIOT_NCOEF],
+                       std::vector<double> (&K)[biot::NKC]);
