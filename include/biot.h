@@ -244,6 +244,9 @@ const char *biot_batch_last_error(const biot_batch *b);
 int biot_batch_add_shot(biot_batch *b, const biot_shot *s, const biot_params *window_params,
                         const float *wavelet, int32_t nt);
 int biot_batch_shot_count(biot_batch *b, int32_t *n);
+/* new model-grid parameters (nx*ny*nz each): every shot's window is cut out of them and its
+ * coefficients rebuilt (an inversion's model update) */
+int biot_batch_set_model(biot_batch *b, const biot_params *model_params);
 /* the per-shot grid (for reading fields or traces through the single-grid calls) */
 int biot_batch_shot_handle(biot_batch *b, int32_t shot, biot_handle **h);
 /* tile shape and persistent grid size (blocks, a multiple of 8) of the batched launches */

@@ -152,6 +152,7 @@ def lib():
         "biot_batch_last_error": ([H], C.c_char_p),
         "biot_batch_add_shot": ([H, C.POINTER(Shot), C.POINTER(Params), FP, i32], C.c_int),
         "biot_batch_shot_count": ([H, C.POINTER(i32)], C.c_int),
+        "biot_batch_set_model": ([H, C.POINTER(Params)], C.c_int),
         "biot_batch_shot_handle": ([H, i32, C.POINTER(H)], C.c_int),
         "biot_batch_set_launch": ([H, i32, i32, i32, i32], C.c_int),
         "biot_batch_get_launch": ([H, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
@@ -189,7 +190,8 @@ def exported_symbols():
             "biot_forward_checkpointed", "biot_misfit_l2", "biot_adjoint",
             "biot_params_coefficients", "biot_params_chain", "biot_batch_create",
             "biot_batch_destroy", "biot_batch_last_error", "biot_batch_add_shot",
-            "biot_batch_shot_count", "biot_batch_shot_handle", "biot_batch_set_launch",
+            "biot_batch_shot_count", "biot_batch_set_model", "biot_batch_shot_handle",
+            "biot_batch_set_launch",
             "biot_batch_get_launch", "biot_batch_autotune", "biot_batch_forward",
             "biot_batch_read_traces", "biot_batch_gradient", "biot_batch_last_ms",
             "biot_batch_gradient_groups",

@@ -66,6 +66,12 @@ class Batch:
         self.shots.append(dict(ncomp=ncomp, nt=len(w), nrec=len(cells), size=(wx, wy, wz),
                                origin=(x0, y0, z0)))
 
+    def set_model(self, params):
+        """Model-grid parameters (dict of the eight arrays, shape (nz, ny, nx)): every shot's
+        window is cut out and its coefficients rebuilt."""
+        st, keep = _lib.params_struct(params, self.n_model)
+        self._check(lib().biot_batch_set_model(self._h, C.byref(st)), "biot_batch_set_model")
+
     def set_launch(self, bx, by, zchunk, blocks):
         self._check(lib().biot_batch_set_launch(self._h, bx, by, zchunk, blocks), "biot_batch_set_launch")
 

@@ -139,7 +139,7 @@ def make_solver(cfg, precision="fp32", order=None, nt=None, with_receivers=True)
                npml=cfg["npml"])
     w = ricker(cfg["f0"], cfg["dt"], nt)
     s.set_source(int(cells(cfg, cfg["src"])[0]), w, cfg["src_fields"])
-    s.set_medium(cfg["medium"])
+    s.set_medium_device(cfg["medium"])  # the same bits as set_medium, built on the GPU
     if with_receivers:
         s.set_receivers(cells(cfg, cfg["rec"]), cfg["rec_fields"], nt)
     return s

@@ -230,6 +230,38 @@ extern "C" int biot_batch_add_shot(biot_batch *b, const biot_shot *s, const biot
   return BIOT_OK;
 }
 
+// New model-grid parameters: each shot's window is cut out and its coefficients rebuilt.
+extern "C" int biot_batch_set_model(biot_batch *b, const biot_params *p)
+{
+  if (!b) return BIOT_ERR_ARG;
+  if (!p || !p->lambda || !p->mu || !p->alpha || !p->M || !p->rho || !p->rhof || !p->C1 || !p->tphi)
+    return bfail(b, BIOT_ERR_ARG, "every model parameter array is required");
+  const double *src[8] = {p->lambda, p->mu, p->alpha, p->M, p->rho, p->rhof, p->C1, p->tphi};
+  const int64_t msy = b->cfg.nx, msz = (int64_t)b->cfg.nx * b->cfg.ny;
+  b->free_gradient();
+  for (size_t i = 0; i < b->shots.size(); ++i) {
+    auto &s = b->shots[i];
+    biot_handle *h = s.h;
+    const int wx = h->g.nx, wy = h->g.ny, wz = h->g.nz;
+    std::vector<double> win[8];
+    for (int q = 0; q < 8; ++q) {
+      win[q].resize((size_t)h->n);
+      for (int k = 0; k < wz; ++k)
+        for (int j = 0; j < wy; ++j) {
+          const double *row = src[q] + (s.x0 + (int64_t)(s.y0 + j) * msy + (int64_t)(s.z0 + k) * msz);
+          std::memcpy(win[q].data() + ((int64_t)k * wy + j) * wx, row, (size_t)wx * sizeof(double));
+        }
+    }
+    const biot_params wp{win[0].data(), win[1].data(), win[2].data(), win[3].data(),
+                         win[4].data(), win[5].data(), win[6].data(), win[7].data()};
+    int rc = biot_set_params(h, &wp);
+    if (rc) return hfail(b, h, rc, "shot parameters");
+  }
+  b->built = false;
+  b->drop_graph();
+  return BIOT_OK;
+}
+
 extern "C" int biot_batch_shot_count(biot_batch *b, int32_t *n)
 {
   if (!b || !n) return BIOT_ERR_ARG;
