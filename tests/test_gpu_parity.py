@@ -119,6 +119,33 @@ def test_half_precision_perturbed_medium_vs_gpu_fp64(precision, bar):
     assert max(errs) <= bar
 
 
+def test_half_precision_full_size_perturbed512():
+    """The 512 x 512 x 384, 3000-step perturbed-medium workload at full size: FP16C (binary16
+    + 8-bit residual on v, tau, p) within 1e-2 of the GPU FP64 path on the 128 x 128 surface
+    receivers; plain FP16 is reported."""
+    import time
+
+    print("perturbed512 full size", flush=True)
+    t0 = time.time()
+    cfg = configs.perturbed512()
+    print(f"  medium built in {time.time() - t0:.0f} s", flush=True)
+    traces = {}
+    for prec in ("fp64", "fp16c", "fp16"):
+        t0 = time.time()
+        s = configs.make_solver(cfg, precision=prec)
+        s.run(cfg["nt"])
+        traces[prec] = s.traces()
+        ms = s.last_run_ms()
+        s.close()
+        print(f"  {prec}: {ms / 1e3:.1f} s loop, {time.time() - t0:.0f} s total", flush=True)
+    ref = traces["fp64"]
+    errs = {p: [rel_l2(traces[p][c], ref[c]) for c in range(3)] for p in ("fp16c", "fp16")}
+    for p, e in errs.items():
+        print(f"  {p} traces rel L2 vs fp64 (vx, vz, p) = " + " ".join(f"{x:.3e}" for x in e))
+    assert np.isfinite(traces["fp16c"]).all() and np.isfinite(traces["fp16"]).all()
+    assert max(errs["fp16c"]) <= 1e-2
+
+
 def _random_state(shape, seed):
     rng = np.random.default_rng(seed)
     return [rng.standard_normal(shape).astype(np.float32).astype(np.float64) for _ in range(13)]
