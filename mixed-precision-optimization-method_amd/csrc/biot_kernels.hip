@@ -166,17 +166,30 @@ __device__ __forceinline__ typename P::A coef(const AT &a, int idx, u32 c)
   return v;
 }
 
-// One folded C-PML update: psi = carry + a D; carry <- e (psi + a D); returns D + psi.
+// One folded C-PML update on a carry held in a register: psi = carry + a D;
+// carry <- e (psi + a D); returns D + psi.
 template <class P>
-__device__ __forceinline__ typename P::A pml(void *base, u32 idx, typename P::A D, typename P::A e,
+__device__ __forceinline__ typename P::A pml(typename P::SC &cv, typename P::A D, typename P::A e,
                                              typename P::A aa)
 {
   using A = typename P::A;
   using SC = typename P::SC;
-  SC &cp = at<SC>(base, idx);
-  const A psi = (A)cp + aa * D;
-  cp = (SC)(e * (psi + aa * D));
+  const A psi = (A)cv + aa * D;
+  cv = (SC)(e * (psi + aa * D));
   return D + psi;
+}
+// the four carries of one slab at o, o + sl, o + 2 sl, o + 3 sl
+template <class SC>
+__device__ __forceinline__ void carry_load(SC (&cv)[4], const void *base, u32 o, u32 sl)
+{
+#pragma unroll
+  for (int m = 0; m < 4; ++m) cv[m] = at<SC>(base, o + (u32)m * sl);
+}
+template <class SC>
+__device__ __forceinline__ void carry_store(const SC (&cv)[4], void *base, u32 o, u32 sl)
+{
+#pragma unroll
+  for (int m = 0; m < 4; ++m) at<SC>(base, o + (u32)m * sl) = cv[m];
 }
 
 // slab position of index i on an axis with widths L (low) and R (high), or -1
@@ -259,6 +272,11 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     wp[m] = L<P, FP>(a, col + (u32)(kb - h + 1 + m) * sz);
   }
 
+  // Every load of a plane is written before the plane's first store.  The compiler cannot
+  // prove the arrays distinct, so it never moves a load above a store; in this order all of
+  // a plane's loads (fields, coefficients, C-PML carries) are in flight together and a
+  // plane costs one memory round trip instead of one per store.
+  using SC = typename P::SC;
   for (int k = kb; k < ke; ++k) {
     const u32 c = col + (u32)k * sz;
     wxz[W - 1] = L<P, FTXZ>(a, c + (u32)(h - 1) * sz);
@@ -282,41 +300,64 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     A z3 = dbw<P, FTYZ, ORDER>(a, c, sy, tyz_c);
     A s3 = dwin<ORDER>(wp);
 
-    // C-PML memory (kernel.cu:188-216): x memory of x1, s1 (half node), y2, z2 (integer)
-    if (px >= 0) {
-      const u32 o = (u32)px + wxs * ((u32)j + ny * (u32)k);
-      x1 = pml<P>(a.carry[0], o, x1, exh, axh);
-      s1 = pml<P>(a.carry[0], o + sl0, s1, exh, axh);
-      y2 = pml<P>(a.carry[0], o + 2 * sl0, y2, exi, axi);
-      z2 = pml<P>(a.carry[0], o + 3 * sl0, z2, exi, axi);
-    }
-    // y memory of x2, z3 (integer), y1, s2 (half)
-    if (py >= 0) {
-      const u32 o = (u32)i + nx * ((u32)py + wys * (u32)k);
-      x2 = pml<P>(a.carry[1], o, x2, eyi, ayi);
-      y1 = pml<P>(a.carry[1], o + sl1, y1, eyh, ayh);
-      s2 = pml<P>(a.carry[1], o + 2 * sl1, s2, eyh, ayh);
-      z3 = pml<P>(a.carry[1], o + 3 * sl1, z3, eyi, ayi);
-    }
-    // z memory of x3, y3 (integer), z1, s3 (half)
+    const A qx0 = L<P, FQX>(a, c), qy0 = L<P, FQY>(a, c), qz0 = L<P, FQZ>(a, c);
+    const A vx0 = L<P, FVX>(a, c), vy0 = L<P, FVY>(a, c), vz0 = L<P, FVZ>(a, c);
+    const A w1x = coef<P>(a, K_W1X, c), w2x = coef<P>(a, K_W2X, c), w3x = coef<P>(a, K_W3X, c);
+    const A w1y = coef<P>(a, K_W1Y, c), w2y = coef<P>(a, K_W2Y, c), w3y = coef<P>(a, K_W3Y, c);
+    const A w1z = coef<P>(a, K_W1Z, c), w2z = coef<P>(a, K_W2Z, c);
+    const A u1x = coef<P>(a, K_U1X, c), u2x = coef<P>(a, K_U2X, c);
+    const A u1y = coef<P>(a, K_U1Y, c), u2y = coef<P>(a, K_U2Y, c);
+    const A u1z = coef<P>(a, K_U1Z, c), u2z = coef<P>(a, K_U2Z, c);
+
+    // C-PML carries of the slabs this cell lies in (kernel.cu:188-216): x memory of x1, s1
+    // (half node), y2, z2 (integer); y memory of x2, y1, s2 (half), z3 (integer); z memory of
+    // x3, y3 (integer), z1, s3 (half)
     const int pz = slab_pos(k, a.nz, a.L[2], a.R[2]);
+    SC cx[4], cy[4], cz[4];
+    u32 ox = 0, oy = 0, oz = 0;
+    A ezi = 0, azi = 0, ezh = 0, azh = 0;
+    if (px >= 0) {
+      ox = (u32)px + wxs * ((u32)j + ny * (u32)k);
+      carry_load(cx, a.carry[0], ox, sl0);
+    }
+    if (py >= 0) {
+      oy = (u32)i + nx * ((u32)py + wys * (u32)k);
+      carry_load(cy, a.carry[1], oy, sl1);
+    }
     if (pz >= 0) {
-      const A ezi = prof<A>(a.prof[2][0], k), azi = prof<A>(a.prof[2][1], k);
-      const A ezh = prof<A>(a.prof[2][2], k), azh = prof<A>(a.prof[2][3], k);
-      const u32 o = (u32)i + nx * ((u32)j + ny * (u32)pz);
-      x3 = pml<P>(a.carry[2], o, x3, ezi, azi);
-      y3 = pml<P>(a.carry[2], o + sl2, y3, ezi, azi);
-      z1 = pml<P>(a.carry[2], o + 2 * sl2, z1, ezh, azh);
-      s3 = pml<P>(a.carry[2], o + 3 * sl2, s3, ezh, azh);
+      ezi = prof<A>(a.prof[2][0], k), azi = prof<A>(a.prof[2][1], k);
+      ezh = prof<A>(a.prof[2][2], k), azh = prof<A>(a.prof[2][3], k);
+      oz = (u32)i + nx * ((u32)j + ny * (u32)pz);
+      carry_load(cz, a.carry[2], oz, sl2);
+    }
+    if (px >= 0) {
+      x1 = pml<P>(cx[0], x1, exh, axh);
+      s1 = pml<P>(cx[1], s1, exh, axh);
+      y2 = pml<P>(cx[2], y2, exi, axi);
+      z2 = pml<P>(cx[3], z2, exi, axi);
+      carry_store(cx, a.carry[0], ox, sl0);
+    }
+    if (py >= 0) {
+      x2 = pml<P>(cy[0], x2, eyi, ayi);
+      y1 = pml<P>(cy[1], y1, eyh, ayh);
+      s2 = pml<P>(cy[2], s2, eyh, ayh);
+      z3 = pml<P>(cy[3], z3, eyi, ayi);
+      carry_store(cy, a.carry[1], oy, sl1);
+    }
+    if (pz >= 0) {
+      x3 = pml<P>(cz[0], x3, ezi, azi);
+      y3 = pml<P>(cz[1], y3, ezi, azi);
+      z1 = pml<P>(cz[2], z1, ezh, azh);
+      s3 = pml<P>(cz[3], s3, ezh, azh);
+      carry_store(cz, a.carry[2], oz, sl2);
     }
 
     // relative fluid velocity (kernel.cu:218-220; qz takes W3y as the reference does) and
     // solid velocity with the stored new q (kernel.cu:223-225)
     const A sx = x1 + x2 + x3, sy_ = y1 + y2 + y3, sz_ = z1 + z2 + z3;
-    const A qx0 = L<P, FQX>(a, c), qy0 = L<P, FQY>(a, c), qz0 = L<P, FQZ>(a, c);
-    const A qx1 = coef<P>(a, K_W1X, c) * qx0 - coef<P>(a, K_W2X, c) * sx - coef<P>(a, K_W3X, c) * s1;
-    const A qy1 = coef<P>(a, K_W1Y, c) * qy0 - coef<P>(a, K_W2Y, c) * sy_ - coef<P>(a, K_W3Y, c) * s2;
-    const A qz1 = coef<P>(a, K_W1Z, c) * qz0 - coef<P>(a, K_W2Z, c) * sz_ - coef<P>(a, K_W3Y, c) * s3;
+    const A qx1 = w1x * qx0 - w2x * sx - w3x * s1;
+    const A qy1 = w1y * qy0 - w2y * sy_ - w3y * s2;
+    const A qz1 = w1z * qz0 - w2z * sz_ - w3y * s3;
     ST<P, FQX>(a, c, qx1);
     ST<P, FQY>(a, c, qy1);
     ST<P, FQZ>(a, c, qz1);
@@ -324,9 +365,9 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     // never a compensated field, so the stored value is the plain storage rounding
     using S = typename P::S;
     const A qx1s = (A)(S)qx1, qy1s = (A)(S)qy1, qz1s = (A)(S)qz1;
-    ST<P, FVX>(a, c, L<P, FVX>(a, c) + coef<P>(a, K_U1X, c) * sx - coef<P>(a, K_U2X, c) * (qx1s - qx0));
-    ST<P, FVY>(a, c, L<P, FVY>(a, c) + coef<P>(a, K_U1Y, c) * sy_ - coef<P>(a, K_U2Y, c) * (qy1s - qy0));
-    ST<P, FVZ>(a, c, L<P, FVZ>(a, c) + coef<P>(a, K_U1Z, c) * sz_ - coef<P>(a, K_U2Z, c) * (qz1s - qz0));
+    ST<P, FVX>(a, c, vx0 + u1x * sx - u2x * (qx1s - qx0));
+    ST<P, FVY>(a, c, vy0 + u1y * sy_ - u2y * (qy1s - qy0));
+    ST<P, FVZ>(a, c, vz0 + u1z * sz_ - u2z * (qz1s - qz0));
 
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) {
@@ -384,6 +425,8 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     wvy[m] = L<P, FVY>(a, col + (u32)(kb - h + 1 + m) * sz);
   }
 
+  // all of a plane's loads before its first store (see velocity_tile)
+  using SC = typename P::SC;
   for (int k = kb; k < ke; ++k) {
     const u32 c = col + (u32)k * sz;
     wvz[W - 1] = L<P, FVZ>(a, c + (u32)(h - 1) * sz);
@@ -407,42 +450,63 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     A uyz = dwin<ORDER>(wvy);
     A uzy = dfw<P, FVZ, ORDER>(a, c, sy, vz_c);
 
-    // C-PML memory (kernel.cu:449-482): carries 4..7 of each axis
+    const A p0 = L<P, FP>(a, c), txx0 = L<P, FTXX>(a, c), tyy0 = L<P, FTYY>(a, c);
+    const A tzz0 = L<P, FTZZ>(a, c), txy0 = L<P, FTXY>(a, c), tyz0 = L<P, FTYZ>(a, c);
+    const A txz0 = L<P, FTXZ>(a, c);
+    const A s1 = coef<P>(a, K_S1, c), s2 = coef<P>(a, K_S2, c), s3 = coef<P>(a, K_S3, c);
+    const A kp1 = coef<P>(a, K_P1, c), kp2 = coef<P>(a, K_P2, c);
+    const A kxy = coef<P>(a, K_SXY, c), kyz = coef<P>(a, K_SYZ, c), kxz = coef<P>(a, K_SXZ, c);
+
+    // C-PML carries 4..7 of each axis (kernel.cu:449-482)
+    const int pz = slab_pos(k, a.nz, a.L[2], a.R[2]);
+    SC cx[4], cy[4], cz[4];
+    u32 ox = 0, oy = 0, oz = 0;
+    A ezi = 0, azi = 0, ezh = 0, azh = 0;
     if (px >= 0) {
-      const u32 o = 4 * sl0 + (u32)px + wxs * ((u32)j + ny * (u32)k);
-      uxx = pml<P>(a.carry[0], o, uxx, exi, axi);
-      wx = pml<P>(a.carry[0], o + sl0, wx, exi, axi);
-      uyx = pml<P>(a.carry[0], o + 2 * sl0, uyx, exh, axh);
-      uzx = pml<P>(a.carry[0], o + 3 * sl0, uzx, exh, axh);
+      ox = 4 * sl0 + (u32)px + wxs * ((u32)j + ny * (u32)k);
+      carry_load(cx, a.carry[0], ox, sl0);
     }
     if (py >= 0) {
-      const u32 o = 4 * sl1 + (u32)i + nx * ((u32)py + wys * (u32)k);
-      uyy = pml<P>(a.carry[1], o, uyy, eyi, ayi);
-      wy = pml<P>(a.carry[1], o + sl1, wy, eyi, ayi);
-      uxy = pml<P>(a.carry[1], o + 2 * sl1, uxy, eyh, ayh);
-      uzy = pml<P>(a.carry[1], o + 3 * sl1, uzy, eyh, ayh);
+      oy = 4 * sl1 + (u32)i + nx * ((u32)py + wys * (u32)k);
+      carry_load(cy, a.carry[1], oy, sl1);
     }
-    const int pz = slab_pos(k, a.nz, a.L[2], a.R[2]);
     if (pz >= 0) {
-      const A ezi = prof<A>(a.prof[2][0], k), azi = prof<A>(a.prof[2][1], k);
-      const A ezh = prof<A>(a.prof[2][2], k), azh = prof<A>(a.prof[2][3], k);
-      const u32 o = 4 * sl2 + (u32)i + nx * ((u32)j + ny * (u32)pz);
-      uzz = pml<P>(a.carry[2], o, uzz, ezi, azi);
-      wz = pml<P>(a.carry[2], o + sl2, wz, ezi, azi);
-      uxz = pml<P>(a.carry[2], o + 2 * sl2, uxz, ezh, azh);
-      uyz = pml<P>(a.carry[2], o + 3 * sl2, uyz, ezh, azh);
+      ezi = prof<A>(a.prof[2][0], k), azi = prof<A>(a.prof[2][1], k);
+      ezh = prof<A>(a.prof[2][2], k), azh = prof<A>(a.prof[2][3], k);
+      oz = 4 * sl2 + (u32)i + nx * ((u32)j + ny * (u32)pz);
+      carry_load(cz, a.carry[2], oz, sl2);
+    }
+    if (px >= 0) {
+      uxx = pml<P>(cx[0], uxx, exi, axi);
+      wx = pml<P>(cx[1], wx, exi, axi);
+      uyx = pml<P>(cx[2], uyx, exh, axh);
+      uzx = pml<P>(cx[3], uzx, exh, axh);
+      carry_store(cx, a.carry[0], ox, sl0);
+    }
+    if (py >= 0) {
+      uyy = pml<P>(cy[0], uyy, eyi, ayi);
+      wy = pml<P>(cy[1], wy, eyi, ayi);
+      uxy = pml<P>(cy[2], uxy, eyh, ayh);
+      uzy = pml<P>(cy[3], uzy, eyh, ayh);
+      carry_store(cy, a.carry[1], oy, sl1);
+    }
+    if (pz >= 0) {
+      uzz = pml<P>(cz[0], uzz, ezi, azi);
+      wz = pml<P>(cz[1], wz, ezi, azi);
+      uxz = pml<P>(cz[2], uxz, ezh, azh);
+      uyz = pml<P>(cz[3], uyz, ezh, azh);
+      carry_store(cz, a.carry[2], oz, sl2);
     }
 
     // pore pressure and stresses (kernel.cu:484-490)
     const A divw = wx + wy + wz;
-    const A s1 = coef<P>(a, K_S1, c), s2 = coef<P>(a, K_S2, c), s3 = coef<P>(a, K_S3, c);
-    ST<P, FP>(a, c, L<P, FP>(a, c) - coef<P>(a, K_P1, c) * (uxx + uyy + uzz) - coef<P>(a, K_P2, c) * divw);
-    ST<P, FTXX>(a, c, L<P, FTXX>(a, c) + s1 * (uyy + uzz) + s2 * uxx + s3 * divw);
-    ST<P, FTYY>(a, c, L<P, FTYY>(a, c) + s1 * (uxx + uzz) + s2 * uyy + s3 * divw);
-    ST<P, FTZZ>(a, c, L<P, FTZZ>(a, c) + s1 * (uxx + uyy) + s2 * uzz + s3 * divw);
-    ST<P, FTXY>(a, c, L<P, FTXY>(a, c) + coef<P>(a, K_SXY, c) * (uxy + uyx));
-    ST<P, FTYZ>(a, c, L<P, FTYZ>(a, c) + coef<P>(a, K_SYZ, c) * (uyz + uzy));
-    ST<P, FTXZ>(a, c, L<P, FTXZ>(a, c) + coef<P>(a, K_SXZ, c) * (uxz + uzx));
+    ST<P, FP>(a, c, p0 - kp1 * (uxx + uyy + uzz) - kp2 * divw);
+    ST<P, FTXX>(a, c, txx0 + s1 * (uyy + uzz) + s2 * uxx + s3 * divw);
+    ST<P, FTYY>(a, c, tyy0 + s1 * (uxx + uzz) + s2 * uyy + s3 * divw);
+    ST<P, FTZZ>(a, c, tzz0 + s1 * (uxx + uyy) + s2 * uzz + s3 * divw);
+    ST<P, FTXY>(a, c, txy0 + kxy * (uxy + uyx));
+    ST<P, FTYZ>(a, c, tyz0 + kyz * (uyz + uzy));
+    ST<P, FTXZ>(a, c, txz0 + kxz * (uxz + uzx));
 
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) {
