@@ -156,29 +156,49 @@ __device__ __forceinline__ typename AdjT<S>::A kcoef(const AT &a, int kidx, u32 
   return v;
 }
 
-// post-PML forward derivative (table entry D, fields of state qf, carries of state qc) and
-// the carry adjoint update
+// Loads of one derivative-table entry: the forward derivative E (fields of state qf) and, in
+// the entry's absorbing slab, the profile values, the forward carry (state qc) and the carry
+// adjoint.  Every kernel below issues all of its loads before its first store: the compiler
+// cannot prove the arrays distinct and never moves a load above a store, so this order keeps
+// a cell's loads in flight together (one memory round trip instead of one per store).
+template <typename A>
+struct PmlIn {
+  A E, e, aa, cf, lc;
+  long long ix;
+};
 template <typename S, int ORDER, class AT>
-__device__ __forceinline__ void pml_adjoint(const AT &a, const Der &D, const CellIdx &ci, int qf,
-                                            int qc, typename AdjT<S>::A lamEp,
-                                            typename AdjT<S>::A &fwdEp, typename AdjT<S>::A &G)
+__device__ __forceinline__ void pml_load(const AT &a, const Der &D, const CellIdx &ci, int qf, int qc,
+                                         PmlIn<typename AdjT<S>::A> &r)
 {
   using A = typename AdjT<S>::A;
   const u32 st[3] = {1u, (u32)a.sy, (u32)a.sz};
-  const A E = diff<S, ORDER>(fw_field(a, qf, D.field), ci.c, st[D.dir], D.fwd);
-  const long long ix = carry_idx(a, D.dir, D.slot, ci);
-  if (ix < 0) {
-    fwdEp = E;
+  r.E = diff<S, ORDER>(fw_field(a, qf, D.field), ci.c, st[D.dir], D.fwd);
+  r.ix = carry_idx(a, D.dir, D.slot, ci);
+  r.e = r.aa = r.cf = r.lc = A(0);
+  if (r.ix >= 0) {
+    r.e = profv<A>(a, D.dir, D.half ? 2 : 0, ci);
+    r.aa = profv<A>(a, D.dir, D.half ? 3 : 1, ci);
+    r.cf = at<A>(fw_carry(a, qc, D.dir), (u32)r.ix);
+    r.lc = at<A>(a.lcarry[D.dir], (u32)r.ix);
+  }
+}
+// post-PML forward derivative fwdEp, derivative adjoint G and the updated carry adjoint r.lc
+template <typename A>
+__device__ __forceinline__ void pml_apply(PmlIn<A> &r, A lamEp, A &fwdEp, A &G)
+{
+  if (r.ix < 0) {
+    fwdEp = r.E;
     G = lamEp;
     return;
   }
-  const A e = profv<A>(a, D.dir, D.half ? 2 : 0, ci);
-  const A aa = profv<A>(a, D.dir, D.half ? 3 : 1, ci);
-  const A cf = at<A>(fw_carry(a, qc, D.dir), (u32)ix);
-  fwdEp = (A(1) + aa) * E + cf;
-  A &lc = at<A>(a.lcarry[D.dir], (u32)ix);
-  G = (A(1) + aa) * lamEp + A(2) * aa * e * lc;
-  lc = lamEp + e * lc;
+  fwdEp = (A(1) + r.aa) * r.E + r.cf;
+  G = (A(1) + r.aa) * lamEp + A(2) * r.aa * r.e * r.lc;
+  r.lc = lamEp + r.e * r.lc;
+}
+template <typename A, class AT>
+__device__ __forceinline__ void pml_store(const AT &a, const Der &D, const PmlIn<A> &r)
+{
+  if (r.ix >= 0) at<A>(a.lcarry[D.dir], (u32)r.ix) = r.lc;
 }
 
 template <typename S, int ORDER, class AT>
@@ -195,6 +215,15 @@ __device__ __forceinline__ void adj_stress_local(const AT &a, int q, u32 t)
   const A S1 = kcoef<S>(a, K_S1, c), S2 = kcoef<S>(a, K_S2, c), S3 = kcoef<S>(a, K_S3, c);
   const A P1 = kcoef<S>(a, K_P1, c), P2 = kcoef<S>(a, K_P2, c);
   const A Sxy = kcoef<S>(a, K_SXY, c), Sxz = kcoef<S>(a, K_SXZ, c), Syz = kcoef<S>(a, K_SYZ, c);
+  PmlIn<A> in[12];
+#pragma unroll
+  for (int d = 0; d < 12; ++d)  // velocities of state q+1 (after the velocity half step), carries of state q
+    pml_load<S, ORDER>(a, TDER[d], ci, q + 1, q, in[d]);
+  constexpr int KG[8] = {K_P1, K_P2, K_S1, K_S2, K_S3, K_SXY, K_SXZ, K_SYZ};
+  double g[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) g[m] = a.grad[KG[m]][c];
+
   A lE[12];
   lE[0] = S2 * ltxx + S1 * (ltyy + ltzz) - P1 * lp;
   lE[1] = S2 * ltyy + S1 * (ltxx + ltzz) - P1 * lp;
@@ -203,24 +232,26 @@ __device__ __forceinline__ void adj_stress_local(const AT &a, int q, u32 t)
   lE[6] = lE[7] = Sxy * ltxy;
   lE[8] = lE[9] = Sxz * ltxz;
   lE[10] = lE[11] = Syz * ltyz;
-  A E[12];
+  A E[12], G[12];
 #pragma unroll
-  for (int d = 0; d < 12; ++d) {
-    A G;
-    // velocities of state q+1 (after the velocity half step), carries of state q
-    pml_adjoint<S, ORDER>(a, TDER[d], ci, q + 1, q, lE[d], E[d], G);
-    at<A>(a.G[d], c) = G;
-  }
+  for (int d = 0; d < 12; ++d) pml_apply(in[d], lE[d], E[d], G[d]);
   // FP64 gradient sums: dJ/dK += lambda(output) x (forward term), kernel.cu:484-490
   const double divu = (double)E[0] + E[1] + E[2], divw = (double)E[3] + E[4] + E[5];
-  a.grad[K_P1][c] += -(double)lp * divu;
-  a.grad[K_P2][c] += -(double)lp * divw;
-  a.grad[K_S1][c] += (double)ltxx * (E[1] + E[2]) + (double)ltyy * (E[0] + E[2]) + (double)ltzz * (E[0] + E[1]);
-  a.grad[K_S2][c] += (double)ltxx * E[0] + (double)ltyy * E[1] + (double)ltzz * E[2];
-  a.grad[K_S3][c] += ((double)ltxx + ltyy + ltzz) * divw;
-  a.grad[K_SXY][c] += (double)ltxy * (E[6] + E[7]);
-  a.grad[K_SXZ][c] += (double)ltxz * (E[8] + E[9]);
-  a.grad[K_SYZ][c] += (double)ltyz * (E[10] + E[11]);
+  g[0] += -(double)lp * divu;
+  g[1] += -(double)lp * divw;
+  g[2] += (double)ltxx * (E[1] + E[2]) + (double)ltyy * (E[0] + E[2]) + (double)ltzz * (E[0] + E[1]);
+  g[3] += (double)ltxx * E[0] + (double)ltyy * E[1] + (double)ltzz * E[2];
+  g[4] += ((double)ltxx + ltyy + ltzz) * divw;
+  g[5] += (double)ltxy * (E[6] + E[7]);
+  g[6] += (double)ltxz * (E[8] + E[9]);
+  g[7] += (double)ltyz * (E[10] + E[11]);
+#pragma unroll
+  for (int d = 0; d < 12; ++d) {
+    at<A>(a.G[d], c) = G[d];
+    pml_store(a, TDER[d], in[d]);
+  }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) a.grad[KG[m]][c] = g[m];
 }
 
 // lambda(v', q') += transposed differences of the stress-side derivative adjoints
@@ -232,15 +263,21 @@ __device__ __forceinline__ void adj_stress_gather(const AT &a, int, u32 t)
   CellIdx ci;
   if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
   const u32 c = ci.c, sx = 1, sy = (u32)a.sy, sz = (u32)a.sz;
-  at<A>(a.lam[0], c) += diffT<A, ORDER>(a.G[0], c, sx, 0) + diffT<A, ORDER>(a.G[6], c, sy, 1) +
-                        diffT<A, ORDER>(a.G[8], c, sz, 1);
-  at<A>(a.lam[1], c) += diffT<A, ORDER>(a.G[1], c, sy, 0) + diffT<A, ORDER>(a.G[7], c, sx, 1) +
-                        diffT<A, ORDER>(a.G[10], c, sz, 1);
-  at<A>(a.lam[2], c) += diffT<A, ORDER>(a.G[2], c, sz, 0) + diffT<A, ORDER>(a.G[9], c, sx, 1) +
-                        diffT<A, ORDER>(a.G[11], c, sy, 1);
-  at<A>(a.lam[3], c) += diffT<A, ORDER>(a.G[3], c, sx, 0);
-  at<A>(a.lam[4], c) += diffT<A, ORDER>(a.G[4], c, sy, 0);
-  at<A>(a.lam[5], c) += diffT<A, ORDER>(a.G[5], c, sz, 0);
+  const A l0 = at<A>(a.lam[0], c) + (diffT<A, ORDER>(a.G[0], c, sx, 0) + diffT<A, ORDER>(a.G[6], c, sy, 1) +
+                                     diffT<A, ORDER>(a.G[8], c, sz, 1));
+  const A l1 = at<A>(a.lam[1], c) + (diffT<A, ORDER>(a.G[1], c, sy, 0) + diffT<A, ORDER>(a.G[7], c, sx, 1) +
+                                     diffT<A, ORDER>(a.G[10], c, sz, 1));
+  const A l2 = at<A>(a.lam[2], c) + (diffT<A, ORDER>(a.G[2], c, sz, 0) + diffT<A, ORDER>(a.G[9], c, sx, 1) +
+                                     diffT<A, ORDER>(a.G[11], c, sy, 1));
+  const A l3 = at<A>(a.lam[3], c) + diffT<A, ORDER>(a.G[3], c, sx, 0);
+  const A l4 = at<A>(a.lam[4], c) + diffT<A, ORDER>(a.G[4], c, sy, 0);
+  const A l5 = at<A>(a.lam[5], c) + diffT<A, ORDER>(a.G[5], c, sz, 0);
+  at<A>(a.lam[0], c) = l0;
+  at<A>(a.lam[1], c) = l1;
+  at<A>(a.lam[2], c) = l2;
+  at<A>(a.lam[3], c) = l3;
+  at<A>(a.lam[4], c) = l4;
+  at<A>(a.lam[5], c) = l5;
 }
 
 template <typename S, int ORDER, class AT>
@@ -251,47 +288,77 @@ __device__ __forceinline__ void adj_velocity_local(const AT &a, int q, u32 t)
   CellIdx ci;
   if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
   const u32 c = ci.c;
-  // per component: lambda_qe = lambda_q' - U2 lambda_v'; lambda_S = U1 lambda_v' - W2 lambda_qe;
-  // lambda_Ds = -W3 lambda_qe (qz takes W3y, kernel.cu:220)
-  A lS[3], lDs[3], lv[3], lqe[3];
-  const int kU1[3] = {K_U1X, K_U1Y, K_U1Z}, kU2[3] = {K_U2X, K_U2Y, K_U2Z};
-  const int kW1[3] = {K_W1X, K_W1Y, K_W1Z}, kW2[3] = {K_W2X, K_W2Y, K_W2Z};
-  const int kW3[3] = {K_W3X, K_W3Y, K_W3Y};
+  constexpr int kU1[3] = {K_U1X, K_U1Y, K_U1Z}, kU2[3] = {K_U2X, K_U2Y, K_U2Z};
+  constexpr int kW1[3] = {K_W1X, K_W1Y, K_W1Z}, kW2[3] = {K_W2X, K_W2Y, K_W2Z};
+  constexpr int kW3[3] = {K_W3X, K_W3Y, K_W3Y};  // qz takes W3y (kernel.cu:220)
+  // loads: adjoint state, coefficients, forward q of states q and q+1, derivative-table
+  // entries, gradient sums
+  A lv[3], lq[3], U1[3], U2[3], W1[3], W2[3], W3[3];
+  double q0[3], q1[3], gU1[3], gU2[3], gW1[3], gW2[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     lv[m] = at<A>(a.lam[m], c);
-    const A lq = at<A>(a.lam[3 + m], c);
-    const A U1 = kcoef<S>(a, kU1[m], c), U2 = kcoef<S>(a, kU2[m], c);
-    const A W1 = kcoef<S>(a, kW1[m], c), W2 = kcoef<S>(a, kW2[m], c), W3 = kcoef<S>(a, kW3[m], c);
-    lqe[m] = lq - U2 * lv[m];
-    lS[m] = U1 * lv[m] - W2 * lqe[m];
-    lDs[m] = -W3 * lqe[m];
-    at<A>(a.lam[3 + m], c) = U2 * lv[m] + W1 * lqe[m];  // lambda_q at step n (lambda_v is unchanged)
+    lq[m] = at<A>(a.lam[3 + m], c);
+    U1[m] = kcoef<S>(a, kU1[m], c), U2[m] = kcoef<S>(a, kU2[m], c);
+    W1[m] = kcoef<S>(a, kW1[m], c), W2[m] = kcoef<S>(a, kW2[m], c), W3[m] = kcoef<S>(a, kW3[m], c);
+    q0[m] = (double)at<S>(fw_field(a, q, 3 + m), c);
+    q1[m] = (double)at<S>(fw_field(a, q + 1, 3 + m), c);
+  }
+  PmlIn<A> in[12];
+#pragma unroll
+  for (int d = 0; d < 12; ++d) pml_load<S, ORDER>(a, VDER[d], ci, q, q, in[d]);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    gU1[m] = a.grad[kU1[m]][c], gU2[m] = a.grad[kU2[m]][c];
+    gW1[m] = a.grad[kW1[m]][c], gW2[m] = a.grad[kW2[m]][c];
+  }
+  double gW3x = a.grad[K_W3X][c], gW3y = a.grad[K_W3Y][c];
+
+  // per component: lambda_qe = lambda_q' - U2 lambda_v'; lambda_S = U1 lambda_v' - W2 lambda_qe;
+  // lambda_Ds = -W3 lambda_qe (qz takes W3y, kernel.cu:220)
+  A lS[3], lDs[3], lqe[3], lqn[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    lqe[m] = lq[m] - U2[m] * lv[m];
+    lS[m] = U1[m] * lv[m] - W2[m] * lqe[m];
+    lDs[m] = -W3[m] * lqe[m];
+    lqn[m] = U2[m] * lv[m] + W1[m] * lqe[m];  // lambda_q at step n (lambda_v is unchanged)
   }
   // derivative order x1 x2 x3 s1 y1 y2 y3 s2 z1 z2 z3 s3
   const A lD[12] = {lS[0], lS[0], lS[0], lDs[0], lS[1], lS[1], lS[1], lDs[1],
                     lS[2], lS[2], lS[2], lDs[2]};
-  A D[12];
+  A D[12], G[12];
 #pragma unroll
-  for (int d = 0; d < 12; ++d) {
-    A G;
-    pml_adjoint<S, ORDER>(a, VDER[d], ci, q, q, lD[d], D[d], G);
-    at<A>(a.G[d], c) = G;
-  }
+  for (int d = 0; d < 12; ++d) pml_apply(in[d], lD[d], D[d], G[d]);
   // FP64 gradient sums with the forward terms of kernel.cu:218-225
   const double Ssum[3] = {(double)D[0] + D[1] + D[2], (double)D[5] + D[4] + D[6],
                           (double)D[9] + D[10] + D[8]};
   const double Ds[3] = {(double)D[3], (double)D[7], (double)D[11]};
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    const double q0 = (double)at<S>(fw_field(a, q, 3 + m), c);
-    const double q1 = (double)at<S>(fw_field(a, q + 1, 3 + m), c);
-    a.grad[kU1[m]][c] += (double)lv[m] * Ssum[m];
-    a.grad[kU2[m]][c] += -(double)lv[m] * (q1 - q0);
-    a.grad[kW1[m]][c] += (double)lqe[m] * q0;
-    a.grad[kW2[m]][c] += -(double)lqe[m] * Ssum[m];
-    a.grad[kW3[m]][c] += -(double)lqe[m] * Ds[m];
+    gU1[m] += (double)lv[m] * Ssum[m];
+    gU2[m] += -(double)lv[m] * (q1[m] - q0[m]);
+    gW1[m] += (double)lqe[m] * q0[m];
+    gW2[m] += -(double)lqe[m] * Ssum[m];
   }
+  gW3x += -(double)lqe[0] * Ds[0];
+  gW3y += -(double)lqe[1] * Ds[1];  // qy's term, then qz's, into the one W3y sum
+  gW3y += -(double)lqe[2] * Ds[2];
+  // stores
+#pragma unroll
+  for (int m = 0; m < 3; ++m) at<A>(a.lam[3 + m], c) = lqn[m];
+#pragma unroll
+  for (int d = 0; d < 12; ++d) {
+    at<A>(a.G[d], c) = G[d];
+    pml_store(a, VDER[d], in[d]);
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    a.grad[kU1[m]][c] = gU1[m], a.grad[kU2[m]][c] = gU2[m];
+    a.grad[kW1[m]][c] = gW1[m], a.grad[kW2[m]][c] = gW2[m];
+  }
+  a.grad[K_W3X][c] = gW3x;
+  a.grad[K_W3Y][c] = gW3y;
 }
 
 // lambda(tau, p) += transposed differences of the velocity-side derivative adjoints
@@ -303,14 +370,21 @@ __device__ __forceinline__ void adj_velocity_gather(const AT &a, int, u32 t)
   CellIdx ci;
   if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
   const u32 c = ci.c, sx = 1, sy = (u32)a.sy, sz = (u32)a.sz;
-  at<A>(a.lam[6], c) += diffT<A, ORDER>(a.G[0], c, sx, 1);
-  at<A>(a.lam[9], c) += diffT<A, ORDER>(a.G[1], c, sy, 0) + diffT<A, ORDER>(a.G[5], c, sx, 0);
-  at<A>(a.lam[10], c) += diffT<A, ORDER>(a.G[2], c, sz, 0) + diffT<A, ORDER>(a.G[9], c, sx, 0);
-  at<A>(a.lam[7], c) += diffT<A, ORDER>(a.G[4], c, sy, 1);
-  at<A>(a.lam[11], c) += diffT<A, ORDER>(a.G[6], c, sz, 0) + diffT<A, ORDER>(a.G[10], c, sy, 0);
-  at<A>(a.lam[8], c) += diffT<A, ORDER>(a.G[8], c, sz, 1);
-  at<A>(a.lam[12], c) += diffT<A, ORDER>(a.G[3], c, sx, 1) + diffT<A, ORDER>(a.G[7], c, sy, 1) +
-                         diffT<A, ORDER>(a.G[11], c, sz, 1);
+  const A l6 = at<A>(a.lam[6], c) + diffT<A, ORDER>(a.G[0], c, sx, 1);
+  const A l9 = at<A>(a.lam[9], c) + (diffT<A, ORDER>(a.G[1], c, sy, 0) + diffT<A, ORDER>(a.G[5], c, sx, 0));
+  const A l10 = at<A>(a.lam[10], c) + (diffT<A, ORDER>(a.G[2], c, sz, 0) + diffT<A, ORDER>(a.G[9], c, sx, 0));
+  const A l7 = at<A>(a.lam[7], c) + diffT<A, ORDER>(a.G[4], c, sy, 1);
+  const A l11 = at<A>(a.lam[11], c) + (diffT<A, ORDER>(a.G[6], c, sz, 0) + diffT<A, ORDER>(a.G[10], c, sy, 0));
+  const A l8 = at<A>(a.lam[8], c) + diffT<A, ORDER>(a.G[8], c, sz, 1);
+  const A l12 = at<A>(a.lam[12], c) + (diffT<A, ORDER>(a.G[3], c, sx, 1) + diffT<A, ORDER>(a.G[7], c, sy, 1) +
+                                       diffT<A, ORDER>(a.G[11], c, sz, 1));
+  at<A>(a.lam[6], c) = l6;
+  at<A>(a.lam[9], c) = l9;
+  at<A>(a.lam[10], c) = l10;
+  at<A>(a.lam[7], c) = l7;
+  at<A>(a.lam[11], c) = l11;
+  at<A>(a.lam[8], c) = l8;
+  at<A>(a.lam[12], c) = l12;
 }
 
 template <typename S, int ORDER, int WHICH, class AT>
