@@ -57,6 +57,7 @@ __constant__ Der TDER[12] = {{0, 0, 0, 4, 0}, {1, 1, 0, 4, 0}, {2, 2, 0, 4, 0}, 
 template <typename S>
 struct AdjT {
   using A = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
+  using C = typename std::conditional<std::is_same<S, float>::value, float, S>::type;  // forward carry
   static constexpr bool scaled = std::is_same<S, _Float16>::value;
 };
 
@@ -178,7 +179,7 @@ __device__ __forceinline__ void pml_load(const AT &a, const Der &D, const CellId
   if (r.ix >= 0) {
     r.e = profv<A>(a, D.dir, D.half ? 2 : 0, ci);
     r.aa = profv<A>(a, D.dir, D.half ? 3 : 1, ci);
-    r.cf = at<A>(fw_carry(a, qc, D.dir), (u32)r.ix);
+    r.cf = (A)at<typename AdjT<S>::C>(fw_carry(a, qc, D.dir), (u32)r.ix);
     r.lc = at<A>(a.lcarry[D.dir], (u32)r.ix);
   }
 }

@@ -58,7 +58,9 @@ struct StepArgs {
 };
 
 inline size_t field_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP32 ? 4 : 2; }
-inline size_t carry_bytes(int prec) { return prec == PREC_FP64 ? 8 : 4; }
+// C-PML carries: binary16 in the FP16 policy (all state 16-bit, as the reference stores every
+// array in half), float in FP32 and FP16C, double in FP64
+inline size_t carry_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP16 ? 2 : 4; }
 
 // Host-side launchers (biot_kernels.hip).
 struct LaunchShape {

@@ -444,7 +444,7 @@ int biot_handle::rescale_state(const float old_scale[NFIELD])
     if (slab[ax] == 0) continue;
     for (int c = 0; c < NCARRY && ok; ++c) {
       const double f = c < 4 ? rt : (c == 5 ? rq : rv);
-      ok = rescale(prec == BIOT_FP64 ? BIOT_FP64 : BIOT_FP32,
+      ok = rescale(prec == BIOT_FP64 ? BIOT_FP64 : prec == BIOT_FP16 ? BIOT_FP16 : BIOT_FP32,
                    (char *)d_carry[ax] + (size_t)c * slab[ax] * csz, nullptr, slab[ax], f);
     }
   }

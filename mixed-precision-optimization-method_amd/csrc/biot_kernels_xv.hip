@@ -37,7 +37,7 @@ struct P32 {
 };
 struct P16 {
   using S = _Float16;
-  using SC = float;
+  using SC = _Float16;
   using A = float;
   static constexpr bool comp = false, scaled = true;
 };
