@@ -34,6 +34,14 @@ class OrConfig(C.Structure):
 _lib = None
 
 
+class HaProblem(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("order", C.c_int),
+                ("prof", (DP * 4) * 3), ("K", DP * 22), ("src_cell", C.c_int64),
+                ("src_mask", C.c_uint), ("wavelet", FP), ("nt_w", C.c_int),
+                ("rec", C.POINTER(C.c_int64)), ("nrec", C.c_int64), ("comp", C.POINTER(C.c_int)),
+                ("ncomp", C.c_int)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -68,6 +76,8 @@ def lib():
         L.or_run_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, FP, C.c_int64, C.c_uint,
                                  C.c_int, C.POINTER(C.c_int64), C.c_uint, DP]
         L.or_num_threads.restype = C.c_int
+        L.ha_forward.argtypes = [C.POINTER(HaProblem), C.c_int, DP]
+        L.ha_adjoint.argtypes = [C.POINTER(HaProblem), C.c_int, C.c_int, DP, DP * 22, DP]
         _lib = L
     return _lib
 
@@ -225,3 +235,73 @@ class OracleRun:
 
 def num_threads():
     return lib().or_num_threads()
+
+
+# ---- FP64 host adjoint (biot_host_adjoint.c; the reference has no adjoint) ----
+
+def host_profiles(nx, ny, nz, dx, dt, npml, v0=4000.0, RC=1e-6):
+    """Per axis (x, y, z) the four double profiles of the folded C-PML carry on the device's
+    FP64 path: e and a = -dt d / 2 at integer nodes, then at half nodes, with d and e taken
+    through the reference's x0.01 float storage (main.cpp:1512-1523) and back x100."""
+    out = []
+    c = np.float32(0.01)
+    for n in (nx, ny, nz):
+        d, d2, e, e2 = pml_profile(n, dx, dt, npml, v0, RC)
+        ds, d2s, es, e2s = [(a * c).astype(np.float32).astype(np.float64) for a in (d, d2, e, e2)]
+        out.append([es * 100.0, -float(dt) * ds * 100.0 * 0.5,
+                    e2s * 100.0, -float(dt) * d2s * 100.0 * 0.5])
+    return out
+
+
+class HostAdjoint:
+    """The FP64 host forward and adjoint of one configuration: grid, order, profiles
+    (host_profiles), the 22 kernel coefficient arrays K (double, enum biot_kcoef order),
+    source (cell, field mask, float wavelet) and receivers (cells, recorded field indices)."""
+
+    def __init__(self, nx, ny, nz, order, prof, K, src_cell, src_mask, wavelet, rec, comp):
+        self.n = nx * ny * nz
+        self._keep = []
+
+        def keep(a, dt):
+            a = np.ascontiguousarray(np.asarray(a, dt).reshape(-1))
+            self._keep.append(a)
+            return a
+        P = HaProblem()
+        P.nx, P.ny, P.nz, P.order = nx, ny, nz, order
+        for ax in range(3):
+            for q in range(4):
+                P.prof[ax][q] = keep(prof[ax][q], np.float64).ctypes.data_as(DP)
+        if len(K) != 22:
+            raise ValueError("22 coefficient arrays expected")
+        for q in range(22):
+            a = keep(K[q], np.float64)
+            if a.size != self.n:
+                raise ValueError(f"coefficient {q} has {a.size} cells, expected {self.n}")
+            P.K[q] = a.ctypes.data_as(DP)
+        P.src_cell, P.src_mask = int(src_cell), int(src_mask)
+        w = keep(wavelet, np.float32)
+        P.wavelet, P.nt_w = w.ctypes.data_as(FP), w.size
+        r = keep(rec, np.int64)
+        P.rec, P.nrec = r.ctypes.data_as(C.POINTER(C.c_int64)), r.size
+        cp = keep(comp, np.int32)
+        P.comp, P.ncomp = cp.ctypes.data_as(C.POINTER(C.c_int)), cp.size
+        self.P, self.nrec, self.ncomp = P, r.size, cp.size
+
+    def forward(self, nt):
+        tr = np.zeros((self.ncomp, nt, self.nrec), np.float64)
+        if lib().ha_forward(C.byref(self.P), int(nt), tr.ctypes.data_as(DP)) != 0:
+            raise RuntimeError("ha_forward failed")
+        return tr
+
+    def adjoint(self, nt, adj, every=10):
+        """dJ/dK (list of 22 arrays) for dJ/d(traces) = adj, and the forward traces."""
+        a = np.ascontiguousarray(np.asarray(adj, np.float64))
+        if a.shape != (self.ncomp, nt, self.nrec):
+            raise ValueError(f"adjoint source shape {a.shape}, expected {(self.ncomp, nt, self.nrec)}")
+        gK = [np.zeros(self.n, np.float64) for _ in range(22)]
+        tr = np.zeros((self.ncomp, nt, self.nrec), np.float64)
+        rc = lib().ha_adjoint(C.byref(self.P), int(nt), int(every), a.ctypes.data_as(DP),
+                              (DP * 22)(*[g.ctypes.data_as(DP) for g in gK]), tr.ctypes.data_as(DP))
+        if rc != 0:
+            raise RuntimeError(f"ha_adjoint failed ({rc})")
+        return gK, tr

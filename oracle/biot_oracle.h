@@ -133,6 +133,28 @@ int or_run_f64(or_state *st, int it0, int nsteps, int nt_total, const float *wav
 
 int or_num_threads(void);
 
+/* ---- FP64 host adjoint (biot_host_adjoint.c; no counterpart in the reference) ---- */
+typedef struct ha_problem {
+  int nx, ny, nz, order;
+  /* per axis (x, y, z): e and a = -dt d / 2 at integer nodes, then at half nodes */
+  const double *prof[3][4];
+  const double *K[22];            /* kernel coefficients (enum biot_kcoef order) */
+  int64_t src_cell;
+  unsigned src_mask;              /* fields the wavelet is added to */
+  const float *wavelet;
+  int nt_w;
+  const int64_t *rec;             /* receiver cells */
+  int64_t nrec;
+  const int *comp;                /* recorded fields, trace component order */
+  int ncomp;
+} ha_problem;
+/* nt steps from rest (source -> V -> T -> record), traces [ncomp][nt][nrec] */
+int ha_forward(const ha_problem *p, int nt, double *traces);
+/* dJ/dK of J with dJ/d(traces) = adj ([ncomp][nt][nrec]), checkpoints every `every` steps;
+ * gK: 22 full-volume arrays (overwritten); traces (optional) receives the forward traces */
+int ha_adjoint(const ha_problem *p, int nt, int every, const double *adj, double *const gK[22],
+               double *traces);
+
 #ifdef __cplusplus
 }
 #endif
