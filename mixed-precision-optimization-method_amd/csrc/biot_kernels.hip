@@ -161,7 +161,9 @@ template <class P, class AT>
 __device__ __forceinline__ typename P::A coef(const AT &a, int idx, u32 c)
 {
   using A = typename P::A;
-  const A v = (A)at<typename P::S>(a.k[idx], c);
+  // coefficients are read once per half step: a nontemporal load keeps them from evicting the
+  // fields' neighbour rows from L2
+  const A v = (A)__builtin_nontemporal_load(&at<typename P::S>(a.k[idx], c));
   if constexpr (P::scaled) return v * a.ks[idx];
   return v;
 }
