@@ -223,7 +223,7 @@ __device__ __forceinline__ void adj_stress_local(const AT &a, int q, u32 t)
   constexpr int KG[8] = {K_P1, K_P2, K_S1, K_S2, K_S3, K_SXY, K_SXZ, K_SYZ};
   double g[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) g[m] = a.grad[KG[m]][c];
+  for (int m = 0; m < 8; ++m) g[m] = (double)at<A>(a.gacc[KG[m]], c);
 
   A lE[12];
   lE[0] = S2 * ltxx + S1 * (ltyy + ltzz) - P1 * lp;
@@ -252,7 +252,7 @@ __device__ __forceinline__ void adj_stress_local(const AT &a, int q, u32 t)
     pml_store(a, TDER[d], in[d]);
   }
 #pragma unroll
-  for (int m = 0; m < 8; ++m) a.grad[KG[m]][c] = g[m];
+  for (int m = 0; m < 8; ++m) at<A>(a.gacc[KG[m]], c) = (A)g[m];
 }
 
 // lambda(v', q') += transposed differences of the stress-side derivative adjoints
@@ -308,12 +308,13 @@ __device__ __forceinline__ void adj_velocity_local(const AT &a, int q, u32 t)
   PmlIn<A> in[12];
 #pragma unroll
   for (int d = 0; d < 12; ++d) pml_load<S, ORDER>(a, VDER[d], ci, q, q, in[d]);
+  auto gl = [&](int kk) { return (double)at<A>(a.gacc[kk], c); };
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    gU1[m] = a.grad[kU1[m]][c], gU2[m] = a.grad[kU2[m]][c];
-    gW1[m] = a.grad[kW1[m]][c], gW2[m] = a.grad[kW2[m]][c];
+    gU1[m] = gl(kU1[m]), gU2[m] = gl(kU2[m]);
+    gW1[m] = gl(kW1[m]), gW2[m] = gl(kW2[m]);
   }
-  double gW3x = a.grad[K_W3X][c], gW3y = a.grad[K_W3Y][c];
+  double gW3x = gl(K_W3X), gW3y = gl(K_W3Y);
 
   // per component: lambda_qe = lambda_q' - U2 lambda_v'; lambda_S = U1 lambda_v' - W2 lambda_qe;
   // lambda_Ds = -W3 lambda_qe (qz takes W3y, kernel.cu:220)
@@ -353,13 +354,14 @@ __device__ __forceinline__ void adj_velocity_local(const AT &a, int q, u32 t)
     at<A>(a.G[d], c) = G[d];
     pml_store(a, VDER[d], in[d]);
   }
+  auto gs = [&](int kk, double v) { at<A>(a.gacc[kk], c) = (A)v; };
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    a.grad[kU1[m]][c] = gU1[m], a.grad[kU2[m]][c] = gU2[m];
-    a.grad[kW1[m]][c] = gW1[m], a.grad[kW2[m]][c] = gW2[m];
+    gs(kU1[m], gU1[m]), gs(kU2[m], gU2[m]);
+    gs(kW1[m], gW1[m]), gs(kW2[m], gW2[m]);
   }
-  a.grad[K_W3X][c] = gW3x;
-  a.grad[K_W3Y][c] = gW3y;
+  gs(K_W3X, gW3x);
+  gs(K_W3Y, gW3y);
 }
 
 // lambda(tau, p) += transposed differences of the velocity-side derivative adjoints
@@ -388,13 +390,35 @@ __device__ __forceinline__ void adj_velocity_gather(const AT &a, int, u32 t)
   at<A>(a.lam[12], c) = l12;
 }
 
+// the float partial sums of a segment into the FP64 gradient sums, cleared for the next one
+template <typename S, int ORDER, class AT>
+__device__ __forceinline__ void adj_flush(const AT &a, u32 t)
+{
+  constexpr int h = ORDER / 2;
+  CellIdx ci;
+  if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
+  const u32 c = ci.c;
+  float part[NKC];
+#pragma unroll
+  for (int k = 0; k < NKC; ++k) part[k] = at<float>(a.gacc[k], c);
+  double tot[NKC];
+#pragma unroll
+  for (int k = 0; k < NKC; ++k) tot[k] = a.grad[k][c];
+#pragma unroll
+  for (int k = 0; k < NKC; ++k) {
+    a.grad[k][c] = tot[k] + (double)part[k];
+    at<float>(a.gacc[k], c) = 0.0f;
+  }
+}
+
 template <typename S, int ORDER, int WHICH, class AT>
 __device__ __forceinline__ void adj_pass(const AT &a, int q, u32 t)
 {
   if constexpr (WHICH == 0) adj_stress_local<S, ORDER>(a, q, t);
   else if constexpr (WHICH == 1) adj_stress_gather<S, ORDER>(a, q, t);
   else if constexpr (WHICH == 2) adj_velocity_local<S, ORDER>(a, q, t);
-  else adj_velocity_gather<S, ORDER>(a, q, t);
+  else if constexpr (WHICH == 3) adj_velocity_gather<S, ORDER>(a, q, t);
+  else adj_flush<S, ORDER>(a, t);
 }
 
 template <typename S, int ORDER, int WHICH>
@@ -594,6 +618,7 @@ hipError_t adj_launch(int which, const AdjArgs &a, int q, hipStream_t s)
     case 1: hipLaunchKernelGGL((k_adj<S, ORDER, 1>), dim3(blocks), dim3(256), 0, s, a, q); break;
     case 2: hipLaunchKernelGGL((k_adj<S, ORDER, 2>), dim3(blocks), dim3(256), 0, s, a, q); break;
     case 3: hipLaunchKernelGGL((k_adj<S, ORDER, 3>), dim3(blocks), dim3(256), 0, s, a, q); break;
+    case 4: hipLaunchKernelGGL((k_adj<S, ORDER, 4>), dim3(blocks), dim3(256), 0, s, a, q); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -607,6 +632,7 @@ hipError_t adj_launch_batch(int which, const AdjBatch &b, int q, hipStream_t s)
     case 1: hipLaunchKernelGGL((k_adj_batch<S, ORDER, 1>), dim3(blocks), dim3(256), 0, s, b, q); break;
     case 2: hipLaunchKernelGGL((k_adj_batch<S, ORDER, 2>), dim3(blocks), dim3(256), 0, s, b, q); break;
     case 3: hipLaunchKernelGGL((k_adj_batch<S, ORDER, 3>), dim3(blocks), dim3(256), 0, s, b, q); break;
+    case 4: hipLaunchKernelGGL((k_adj_batch<S, ORDER, 4>), dim3(blocks), dim3(256), 0, s, b, q); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -627,12 +653,13 @@ struct BatchRun {
   hipError_t run(int which) { return adj_launch_batch<S, O>(which, b, q, s); }
 };
 
-// the four passes, in order, for the storage type of prec
+// passes first..last-1 (0-3: the adjoint step, 4: the flush), in order, for the storage
+// type of prec
 template <class F>
-hipError_t four_passes(int prec, int order, F f)
+hipError_t passes(int prec, int order, F f, int first, int last)
 {
   if (order != 2 && order != 4) return hipErrorInvalidValue;
-  for (int which = 0; which < 4; ++which) {
+  for (int which = first; which < last; ++which) {
     hipError_t e;
     if (prec == PREC_FP32) e = order == 2 ? f.template run<float, 2>(which) : f.template run<float, 4>(which);
     else if (prec == PREC_FP64) e = order == 2 ? f.template run<double, 2>(which) : f.template run<double, 4>(which);
@@ -648,13 +675,25 @@ hipError_t four_passes(int prec, int order, F f)
 hipError_t launch_adjoint_step(int prec, int order, const AdjArgs &a, int q, hipStream_t s)
 {
   if (a.nupd == 0) return hipSuccess;
-  return four_passes(prec, order, SingleRun{a, q, s});
+  return passes(prec, order, SingleRun{a, q, s}, 0, 4);
+}
+
+hipError_t launch_adjoint_flush(int prec, int order, const AdjArgs &a, hipStream_t s)
+{
+  if (a.nupd == 0 || prec == PREC_FP64) return hipSuccess;
+  return passes(prec, order, SingleRun{a, 0, s}, 4, 5);
+}
+
+hipError_t launch_adjoint_flush_batch(int prec, int order, const AdjBatch &b, hipStream_t s)
+{
+  if (b.total_blocks <= 0 || prec == PREC_FP64) return hipSuccess;
+  return passes(prec, order, BatchRun{b, 0, s}, 4, 5);
 }
 
 hipError_t launch_adjoint_step_batch(int prec, int order, const AdjBatch &b, int q, hipStream_t s)
 {
   if (b.total_blocks <= 0) return hipSuccess;
-  return four_passes(prec, order, BatchRun{b, q, s});
+  return passes(prec, order, BatchRun{b, q, s}, 0, 4);
 }
 
 hipError_t launch_adjoint_inject(int prec, const InjectArgs &a, int it, hipStream_t s)

@@ -20,6 +20,10 @@ struct AdjArgs {
   void *lcarry[3];           // adjoint carries (arithmetic type), forward layout
   void *G[12];               // derivative adjoints of the current half step (arithmetic type)
   double *grad[NKC];         // dJ/dK in storage units (multiply by the handle's kfac), FP64
+  // per-step accumulation of dJ/dK: float partial sums over one checkpoint segment (FP32 and
+  // FP16 handles; launch_adjoint_flush adds them into grad and clears them), grad itself on
+  // FP64 handles
+  void *gacc[NKC];
   const void *k[NKC];        // kernel coefficients (storage type)
   float ks[NKC];             // per-array coefficient scale (16-bit storage), else 1
   const void *prof[3][4];    // e_int, a_int, e_half, a_half per axis (arithmetic type)
@@ -45,6 +49,8 @@ struct InjectArgs {
 // step q; prec is the storage precision (FP32, FP64 or FP16)
 hipError_t launch_adjoint_step(int prec, int order, const AdjArgs &a, int q, hipStream_t s);
 hipError_t launch_adjoint_inject(int prec, const InjectArgs &a, int it, hipStream_t s);
+// grad += gacc, gacc = 0 over the update box (no-op on FP64 handles, which sum into grad)
+hipError_t launch_adjoint_flush(int prec, int order, const AdjArgs &a, hipStream_t s);
 // J += 1/2 sum (d - obs)^2 (FP64), res = d - obs (res may be null); d is double when
 // d_double, else float
 hipError_t launch_misfit(const void *d, bool d_double, const float *obs, float *res, int64_t n,
@@ -58,6 +64,7 @@ struct AdjBatch {
   int nshots, total_blocks;
 };
 hipError_t launch_adjoint_step_batch(int prec, int order, const AdjBatch &b, int q, hipStream_t s);
+hipError_t launch_adjoint_flush_batch(int prec, int order, const AdjBatch &b, hipStream_t s);
 hipError_t launch_adjoint_inject_batch(int prec, const InjectArgs *tab, int nshots, int64_t max_nrec,
                                        int max_ncomp, int it, hipStream_t s);
 

@@ -150,6 +150,7 @@ extern "C" int biot_adjoint(biot_handle *h, const float *adj_source, double *con
   if (rc) return rc;
   AHIP(h, hipMemsetAsync(h->d_lam, 0, h->lam_bytes(), h->stream));
   AHIP(h, hipMemsetAsync(h->d_grad, 0, (size_t)NKC * n * sizeof(double), h->stream));
+  if (h->d_gpart) AHIP(h, hipMemsetAsync(h->d_gpart, 0, (size_t)NKC * n * sizeof(float), h->stream));
   AHIP(h, hipMemcpyAsync(h->d_adjsrc, adj_source, (size_t)m * sizeof(float), hipMemcpyHostToDevice, h->stream));
 
   AdjArgs A;
@@ -184,6 +185,7 @@ extern "C" int biot_adjoint(biot_handle *h, const float *adj_source, double *con
       AHIP(h, launch_adjoint_inject(h->prec, I, it, h->stream));
       AHIP(h, launch_adjoint_step(h->prec, order, A, it - n0, h->stream));
     }
+    AHIP(h, launch_adjoint_flush(h->prec, order, A, h->stream));  // the segment's float partial sums
   }
   AHIP(h, hipEventRecord(h->ev1, h->stream));
   AHIP(h, hipEventSynchronize(h->ev1));

@@ -512,7 +512,8 @@ size_t shot_gradient_bytes(const biot_handle *h, int nseg, int every)
 {
   const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
   return h->state_bytes() * (size_t)(nseg + every + 1) + h->lam_bytes() +
-         (size_t)NKC * h->n * sizeof(double) + (size_t)8 * h->n * sizeof(double) +
+         (size_t)NKC * h->n * (sizeof(double) + (h->prec == BIOT_FP64 ? 0 : sizeof(float))) +
+         (size_t)8 * h->n * sizeof(double) +
          (size_t)2 * std::max<int64_t>(m, 1) * sizeof(float);
 }
 
@@ -611,6 +612,7 @@ int gradient_group(biot_batch *b, int s0, int s1, int nt, int every, const float
                           h->d_adjsrc, m, b->d_J, b->stream));
     BHIP(b, hipMemsetAsync(h->d_lam, 0, h->lam_bytes(), b->stream));
     BHIP(b, hipMemsetAsync(h->d_grad, 0, (size_t)NKC * h->n * sizeof(double), b->stream));
+    if (h->d_gpart) BHIP(b, hipMemsetAsync(h->d_gpart, 0, (size_t)NKC * h->n * sizeof(float), b->stream));
   }
   // ---- segment recompute and adjoint sweep, all shots of the group per launch
   const AdjBatch ab{b->d_adj, b->d_blocks, S, b->adj_blocks};
@@ -631,6 +633,7 @@ int gradient_group(biot_batch *b, int s0, int s1, int nt, int every, const float
       BHIP(b, launch_adjoint_inject_batch(prec, b->d_inj, S, b->max_nrec, b->max_ncomp, it, b->stream));
       BHIP(b, launch_adjoint_step_batch(prec, order, ab, it - n0, b->stream));
     }
+    BHIP(b, launch_adjoint_flush_batch(prec, order, ab, b->stream));
   }
   // ---- chain rule per shot into the model-grid gradient
   for (int i = s0; i < s1; ++i) {

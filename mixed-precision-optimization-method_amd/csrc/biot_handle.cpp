@@ -94,6 +94,7 @@ int biot_handle::alloc_adjoint(int every)
   };
   if (!get(&d_slots, state_bytes() * (size_t)(every + 1)) || !get(&d_lam, lam_bytes()) ||
       !get((void **)&d_grad, (size_t)NKC * n * sizeof(double)) ||
+      (prec != BIOT_FP64 && !get((void **)&d_gpart, (size_t)NKC * n * sizeof(float))) ||
       !get((void **)&d_adjsrc, (size_t)std::max<int64_t>(m, 1) * sizeof(float))) {
     err = "adjoint buffers do not fit in device memory";
     return BIOT_ERR_NOMEM;
@@ -120,6 +121,7 @@ void biot_handle::adj_args(AdjArgs &A) const
   for (int d = 0; d < 12; ++d) A.G[d] = lp + (size_t)d * n * asz;
   for (int k = 0; k < NKC; ++k) {
     A.grad[k] = d_grad + (size_t)k * n;
+    A.gacc[k] = prec == BIOT_FP64 ? (void *)A.grad[k] : (void *)(d_gpart + (size_t)k * n);
     A.k[k] = (const char *)d_coef + (size_t)k * n * esz;
     A.ks[k] = ks[k];
   }
@@ -160,7 +162,7 @@ void biot_handle::free_adjoint()
     if (p) (void)hipFree(p);
   ckpt.clear();
   ckpt_every = ckpt_nt = 0;
-  for (void **p : {&d_slots, &d_lam, (void **)&d_grad, (void **)&d_adjsrc})
+  for (void **p : {&d_slots, &d_lam, (void **)&d_grad, (void **)&d_gpart, (void **)&d_adjsrc})
     if (*p) {
       (void)hipFree(*p);
       *p = nullptr;

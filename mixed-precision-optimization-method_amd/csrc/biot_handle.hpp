@@ -78,6 +78,7 @@ struct biot_handle {
   void *d_slots = nullptr;                   // segment states (ckpt_every + 1)
   void *d_lam = nullptr;                     // adjoint fields, carries, derivative adjoints
   double *d_grad = nullptr;                  // 22 FP64 coefficient gradients
+  float *d_gpart = nullptr;                  // 22 float per-segment partial sums (not FP64 handles)
   float *d_adjsrc = nullptr;                 // adjoint source traces
   size_t state_bytes() const;
   size_t state_off(int piece) const;  // 0 fields, 1..3 carries of x, y, z, 4 total
