@@ -46,6 +46,10 @@ struct StepArgs {
   void *f[NFIELD];           // field storage
   int8_t *lo[NFIELD];        // FP16C residuals (null for other modes / fields)
   const void *k[NKC];        // coefficient arrays (field storage type)
+  // the same coefficients packed per cell (storage type): kv holds K[0..13] of the velocity
+  // half step, kt K[14..21] of the stress half step, cell c at element c*14 / c*8 (addressed
+  // in 64 bits: these arrays may span more than 4 GiB)
+  const void *kv, *kt;
   float ks[NKC];             // per-array scale (binary16 storage), 1 otherwise
   void *carry[3];            // per axis: NCARRY arrays of slab[ax] elements (carry type)
   int64_t slab[3];           // elements per carry array per axis
@@ -149,6 +153,8 @@ hipError_t launch_set_counter(int *it, int v, hipStream_t s);
 // multiply a stored array by f (lo: its FP16C residuals or null); carries use prec of the
 // carry type, i.e. pass PREC_FP32 for FP16/FP16C carries
 hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
+// per-cell packed copies kv (K[0..13]) and kt (K[14..21]) of the 22 coefficient arrays
+hipError_t launch_pack_coef(int prec, const void *soa, void *kv, void *kt, int64_t n, hipStream_t s);
 hipError_t launch_record(int prec, const RecordArgs &a, hipStream_t s);
 
 }  // namespace biot

@@ -183,6 +183,7 @@ biot_handle::~biot_handle()
   if (d_fields) (void)hipFree(d_fields);
   if (d_lo) (void)hipFree(d_lo);
   if (d_coef) (void)hipFree(d_coef);
+  if (d_kpack) (void)hipFree(d_kpack);
   for (auto *p : d_carry)
     if (p) (void)hipFree(p);
   for (auto &ax : d_prof)
@@ -216,6 +217,8 @@ void biot_handle::fill_args(StepArgs &a) const
     a.k[k] = (const char *)d_coef + (size_t)k * n * esz;
     a.ks[k] = ks[k];
   }
+  a.kv = d_kpack;
+  a.kt = (const char *)d_kpack + (size_t)14 * n * esz;
   for (int ax = 0; ax < 3; ++ax) {
     a.carry[ax] = d_carry[ax];
     a.slab[ax] = slab[ax];
@@ -248,7 +251,8 @@ int biot_handle::allocate()
     device_bytes += (int64_t)bytes;
     return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
   };
-  bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, (size_t)NKC * n * esz);
+  bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, (size_t)NKC * n * esz) &&
+            alloc(&d_kpack, (size_t)NKC * n * esz);
   if (ok && prec == BIOT_FP16C) ok = alloc((void **)&d_lo, (size_t)NFIELD * n);
   for (int ax = 0; ax < 3 && ok; ++ax) ok = alloc(&d_carry[ax], (size_t)NCARRY * slab[ax] * csz);
   const int dims[3] = {g.nx, g.ny, g.nz};
@@ -515,7 +519,20 @@ int biot_handle::upload_from_ref(const float *d_ref)
     err = "coefficient conversion failed";
     return BIOT_ERR_HIP;
   }
+  if ((rc = pack_coefficients())) return rc;
   have_coef = true;
+  return BIOT_OK;
+}
+
+int biot_handle::pack_coefficients()
+{
+  const void *kv = d_kpack;
+  void *kt = (char *)d_kpack + (size_t)14 * n * esz;
+  if (launch_pack_coef(prec, d_coef, (void *)kv, kt, n, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess) {
+    err = "coefficient packing failed";
+    return BIOT_ERR_HIP;
+  }
   return BIOT_OK;
 }
 
@@ -567,6 +584,7 @@ int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
       return BIOT_ERR_HIP;
     }
   }
+  if ((rc = pack_coefficients())) return rc;
   have_coef = true;
   return BIOT_OK;
 }

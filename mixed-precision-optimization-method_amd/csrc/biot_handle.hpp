@@ -24,6 +24,7 @@ struct biot_handle {
   void *d_fields = nullptr;          // 13 * n
   int8_t *d_lo = nullptr;            // FP16C residuals, 13 * n (compensated fields used)
   void *d_coef = nullptr;            // 22 * n
+  void *d_kpack = nullptr;           // the coefficients packed per cell (velocity, then stress block)
   void *d_carry[3] = {nullptr, nullptr, nullptr};
   void *d_prof[3][4] = {};           // e_int, a_int, e_half, a_half per axis
   int64_t slab[3] = {0, 0, 0};
@@ -65,6 +66,7 @@ struct biot_handle {
   int allocate();
   int upload_profiles();
   int upload_coefficients();
+  int pack_coefficients();           // d_coef -> d_kpack
   int upload_kernel_coefs(const std::vector<double> (&K)[biot::NKC]);
   int upload_from_ref(const float *d_ref);      // 23 reference arrays on the device
   void set_scales(const double amaxK[biot::NKC], double resp);

@@ -157,15 +157,58 @@ __device__ __forceinline__ typename P::A dbw(const AT &a, u32 c, u32 s, typename
   }
 }
 
-template <class P, class AT>
-__device__ __forceinline__ typename P::A coef(const AT &a, int idx, u32 c)
+// The N coefficients of cell c from a packed block.  The block stores the values of a cell in
+// 16-byte groups, group-major: group g (values g*G .. g*G+G-1, G = 16 / sizeof(S)) is an array
+// of n 16-byte records at base + g*n*16, and the last N mod G values form a tail array of
+// (N mod G)*sizeof(S)-byte records after the full groups.  Every load of a wavefront is thus
+// one contiguous run (a dwordx4, or the tail's dwordx3/x2, per lane); addresses are 64-bit
+// since a block may span more than 4 GiB.  Each value is multiplied by its array's scale.
+typedef uint32_t dw4_t __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t dw3_t __attribute__((ext_vector_type(3), aligned(4)));
+typedef uint32_t dw2_t __attribute__((ext_vector_type(2), aligned(4)));
+template <int ND>
+__device__ __forceinline__ void ld_rec(const char *p, uint32_t *d)
 {
+  if constexpr (ND == 4) {
+    const dw4_t v = __builtin_nontemporal_load(reinterpret_cast<const dw4_t *>(p));
+    d[0] = v.x, d[1] = v.y, d[2] = v.z, d[3] = v.w;
+  } else if constexpr (ND == 3) {
+    const dw3_t v = __builtin_nontemporal_load(reinterpret_cast<const dw3_t *>(p));
+    d[0] = v.x, d[1] = v.y, d[2] = v.z;
+  } else if constexpr (ND == 2) {
+    const dw2_t v = __builtin_nontemporal_load(reinterpret_cast<const dw2_t *>(p));
+    d[0] = v.x, d[1] = v.y;
+  } else if constexpr (ND == 1) {
+    d[0] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+  }
+}
+template <class P, int N, class AT>
+__device__ __forceinline__ void coef_block(const AT &a, const void *base, u32 c, int k0,
+                                           typename P::A (&out)[N])
+{
+  using S = typename P::S;
   using A = typename P::A;
-  // coefficients are read once per half step: a nontemporal load keeps them from evicting the
-  // fields' neighbour rows from L2
-  const A v = (A)__builtin_nontemporal_load(&at<typename P::S>(a.k[idx], c));
-  if constexpr (P::scaled) return v * a.ks[idx];
-  return v;
+  constexpr int G = 16 / (int)sizeof(S);
+  constexpr int NG = N / G, TAIL = N % G;
+  constexpr int TB = TAIL * (int)sizeof(S);
+  static_assert(TB % 4 == 0, "tail records must be whole dwords");
+  const char *b = reinterpret_cast<const char *>(base);
+  const uint64_t n16 = (uint64_t)a.nx * a.ny * a.nz * 16u;
+  uint32_t d[(N * (int)sizeof(S) + 3) / 4];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) ld_rec<4>(b + g * n16 + (uint64_t)c * 16u, d + 4 * g);
+  if constexpr (TAIL > 0) ld_rec<TB / 4>(b + NG * n16 + (uint64_t)c * TB, d + 4 * NG);
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    if constexpr (std::is_same<S, _Float16>::value) {
+      const uint16_t bits = (uint16_t)(d[e >> 1] >> (16 * (e & 1)));
+      out[e] = (A)__builtin_bit_cast(_Float16, bits) * a.ks[k0 + e];
+    } else if constexpr (std::is_same<S, float>::value) {
+      out[e] = __builtin_bit_cast(float, d[e]);
+    } else {
+      out[e] = __builtin_bit_cast(double, (uint64_t)d[2 * e] | ((uint64_t)d[2 * e + 1] << 32));
+    }
+  }
 }
 
 // One folded C-PML update on a carry held in a register: psi = carry + a D;
@@ -304,12 +347,14 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
 
     const A qx0 = L<P, FQX>(a, c), qy0 = L<P, FQY>(a, c), qz0 = L<P, FQZ>(a, c);
     const A vx0 = L<P, FVX>(a, c), vy0 = L<P, FVY>(a, c), vz0 = L<P, FVZ>(a, c);
-    const A w1x = coef<P>(a, K_W1X, c), w2x = coef<P>(a, K_W2X, c), w3x = coef<P>(a, K_W3X, c);
-    const A w1y = coef<P>(a, K_W1Y, c), w2y = coef<P>(a, K_W2Y, c), w3y = coef<P>(a, K_W3Y, c);
-    const A w1z = coef<P>(a, K_W1Z, c), w2z = coef<P>(a, K_W2Z, c);
-    const A u1x = coef<P>(a, K_U1X, c), u2x = coef<P>(a, K_U2X, c);
-    const A u1y = coef<P>(a, K_U1Y, c), u2y = coef<P>(a, K_U2Y, c);
-    const A u1z = coef<P>(a, K_U1Z, c), u2z = coef<P>(a, K_U2Z, c);
+    A kv[14];  // K[0..13]: U1 xyz, U2 xyz, W1 xyz, W2 xyz, W3x, W3y
+    coef_block<P, 14>(a, a.kv, c, 0, kv);
+    const A w1x = kv[K_W1X], w2x = kv[K_W2X], w3x = kv[K_W3X];
+    const A w1y = kv[K_W1Y], w2y = kv[K_W2Y], w3y = kv[K_W3Y];
+    const A w1z = kv[K_W1Z], w2z = kv[K_W2Z];
+    const A u1x = kv[K_U1X], u2x = kv[K_U2X];
+    const A u1y = kv[K_U1Y], u2y = kv[K_U2Y];
+    const A u1z = kv[K_U1Z], u2z = kv[K_U2Z];
 
     // C-PML carries of the slabs this cell lies in (kernel.cu:188-216): x memory of x1, s1
     // (half node), y2, z2 (integer); y memory of x2, y1, s2 (half), z3 (integer); z memory of
@@ -455,9 +500,11 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     const A p0 = L<P, FP>(a, c), txx0 = L<P, FTXX>(a, c), tyy0 = L<P, FTYY>(a, c);
     const A tzz0 = L<P, FTZZ>(a, c), txy0 = L<P, FTXY>(a, c), tyz0 = L<P, FTYZ>(a, c);
     const A txz0 = L<P, FTXZ>(a, c);
-    const A s1 = coef<P>(a, K_S1, c), s2 = coef<P>(a, K_S2, c), s3 = coef<P>(a, K_S3, c);
-    const A kp1 = coef<P>(a, K_P1, c), kp2 = coef<P>(a, K_P2, c);
-    const A kxy = coef<P>(a, K_SXY, c), kyz = coef<P>(a, K_SYZ, c), kxz = coef<P>(a, K_SXZ, c);
+    A kt[8];  // K[14..21]: P1, P2, S1, S2, S3, Sxy, Sxz, Syz
+    coef_block<P, 8>(a, a.kt, c, K_P1, kt);
+    const A s1 = kt[K_S1 - K_P1], s2 = kt[K_S2 - K_P1], s3 = kt[K_S3 - K_P1];
+    const A kp1 = kt[0], kp2 = kt[K_P2 - K_P1];
+    const A kxy = kt[K_SXY - K_P1], kyz = kt[K_SYZ - K_P1], kxz = kt[K_SXZ - K_P1];
 
     // C-PML carries 4..7 of each axis (kernel.cu:449-482)
     const int pz = slab_pos(k, a.nz, a.L[2], a.R[2]);
@@ -656,6 +703,30 @@ __global__ void k_rescale(S *p, int8_t *lo, int64_t n, double f)
   }
 }
 
+// packing of the coefficient arrays into the velocity block (K[0..13]) and the stress block
+// (K[14..21]), each in the group-major 16-byte layout coef_block reads
+template <typename S>
+__device__ __forceinline__ void pack_block(const S *soa, S *blk, int k0, int N, int64_t n, int64_t c)
+{
+  constexpr int G = 16 / (int)sizeof(S);
+  const int NG = N / G, T = N % G;
+  for (int e = 0; e < N; ++e) {
+    const int g = e / G;
+    const S v = soa[(int64_t)(k0 + e) * n + c];
+    if (g < NG) blk[(int64_t)g * n * G + c * G + (e % G)] = v;
+    else blk[(int64_t)NG * n * G + c * T + (e - NG * G)] = v;
+  }
+}
+template <typename S>
+__global__ void k_pack_coef(const S *soa, S *kv, S *kt, int64_t n)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    pack_block(soa, kv, 0, 14, n, c);
+    pack_block(soa, kt, 14, 8, n, c);
+  }
+}
+
 template <class P, int ORDER, bool VEL>
 hipError_t launch_step(const StepArgs &in, LaunchShape sh, hipStream_t s)
 {
@@ -807,6 +878,29 @@ hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hi
       break;
     case PREC_FP64:
       hipLaunchKernelGGL(k_rescale<double>, dim3(blocks), dim3(256), 0, s, (double *)p, lo, n, f);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_coef(int prec, const void *soa, void *kv, void *kt, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+  switch (prec) {
+    case PREC_FP32:
+      hipLaunchKernelGGL(k_pack_coef<float>, dim3(blocks), dim3(256), 0, s, (const float *)soa, (float *)kv,
+                         (float *)kt, n);
+      break;
+    case PREC_FP16:
+    case PREC_FP16C:
+      hipLaunchKernelGGL(k_pack_coef<_Float16>, dim3(blocks), dim3(256), 0, s, (const _Float16 *)soa,
+                         (_Float16 *)kv, (_Float16 *)kt, n);
+      break;
+    case PREC_FP64:
+      hipLaunchKernelGGL(k_pack_coef<double>, dim3(blocks), dim3(256), 0, s, (const double *)soa, (double *)kv,
+                         (double *)kt, n);
       break;
     default: return hipErrorInvalidValue;
   }
