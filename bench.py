@@ -237,6 +237,28 @@ def kernel_probe(steps):
     print(json.dumps(out))
 
 
+def stream_copy_rate(mib=1024, reps=20):
+    """HBM rate (GB/s, bytes read + written) of torch's device copy of `mib` MiB: what a plain
+    streaming kernel reaches on this GPU."""
+    import torch
+
+    n = mib * 2 ** 20 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB (torch's copy kernel), repeated 5 times:
     the HBM-traffic counters of this dispatch calibrate FETCH_SIZE / WRITE_SIZE."""
@@ -248,7 +270,17 @@ def copy_calibration(mib):
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4}))
+    # the same copy timed (20 repetitions after the 5 above): the HBM rate a plain
+    # streaming kernel reaches on this GPU, for comparison with the stencil kernels
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4,
+                      "copy_ms": round(ms, 4), "copy_GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}))
 
 
 def main():
@@ -318,6 +350,7 @@ def main():
         "stress_kernel": {"kernel": "k_stress<P32,4>", "achieved_GBs": round(ach_t, 1),
                           "frac": round(ach_t / HBM_PEAK_GBS, 4), "algorithmic_bytes": bt,
                           "mean_launch_ms": round(r32["kernel_ms_t"], 5)},
+        "stream_copy_GBs": round(stream_copy_rate(), 1),
         "step_roofline": {"achieved_GBs": round(ach_step, 1), "frac": round(ach_step / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes_per_step": bstep,
                           "bytes_per_cell_update": round(bstep / upd, 2)},
