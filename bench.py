@@ -237,26 +237,19 @@ def kernel_probe(steps):
     print(json.dumps(out))
 
 
-def stream_copy_rate(mib=1024, reps=20):
+def stream_copy_rate(mib=1024):
     """HBM rate (GB/s, bytes read + written) of torch's device copy of `mib` MiB: what a plain
-    streaming kernel reaches on this GPU."""
-    import torch
+    streaming kernel reaches on this GPU.  Measured in a child process (`--copy-calib`) before
+    the solver library starts: torch ships its own HIP runtime, and the two do not share one
+    process's GPU."""
+    import subprocess
 
-    n = mib * 2 ** 20 // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del a, b
-    torch.cuda.empty_cache()
-    return 2 * n * 4 / (ms * 1e-3) / 1e9
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-calib", str(mib)],
+                       capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        log(f"[bench] copy reference failed: {r.stderr.strip()[-300:]}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"]
 
 
 def copy_calibration(mib):
@@ -312,6 +305,7 @@ def main():
 
     from biot_amd import configs
 
+    copy_gbs = stream_copy_rate()  # before this process touches the GPU
     cfg = configs.layered256(order=4, nt=args.warmup + args.steps, n=args.size)
     n = cfg["nx"] * cfg["ny"] * cfg["nz"]
     r32 = run_precision(cfg, "fp32", args.steps, args.warmup)
@@ -350,7 +344,7 @@ def main():
         "stress_kernel": {"kernel": "k_stress<P32,4>", "achieved_GBs": round(ach_t, 1),
                           "frac": round(ach_t / HBM_PEAK_GBS, 4), "algorithmic_bytes": bt,
                           "mean_launch_ms": round(r32["kernel_ms_t"], 5)},
-        "stream_copy_GBs": round(stream_copy_rate(), 1),
+        "stream_copy_GBs": copy_gbs,
         "step_roofline": {"achieved_GBs": round(ach_step, 1), "frac": round(ach_step / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes_per_step": bstep,
                           "bytes_per_cell_update": round(bstep / upd, 2)},

@@ -125,7 +125,8 @@ __device__ __forceinline__ int slab_pos(int i, int n, int Lw, int Rw)
   return -1;
 }
 
-// carry element index of slot `slot` on axis `ax` at cell ci, or -1 outside that slab
+// carry element index of slot `slot` on axis `ax` at cell ci, or -1 outside that slab (records
+// of four carries per slab cell: slots 0..3, then 4..7)
 template <class AT>
 __device__ __forceinline__ long long carry_idx(const AT &a, int ax, int slot, const CellIdx &ci)
 {
@@ -138,7 +139,7 @@ __device__ __forceinline__ long long carry_idx(const AT &a, int ax, int slot, co
   if (ax == 0) o = (u32)p + w * ((u32)ci.j + (u32)a.ny * (u32)ci.k);
   else if (ax == 1) o = (u32)ci.i + (u32)a.nx * ((u32)p + w * (u32)ci.k);
   else o = (u32)ci.i + (u32)a.nx * ((u32)ci.j + (u32)a.ny * (u32)p);
-  return (long long)((u32)slot * (u32)a.slab[ax] + o);
+  return (long long)((u32)(slot >> 2) * 4u * (u32)a.slab[ax] + 4u * o + (u32)(slot & 3));
 }
 
 template <typename A, class AT>

@@ -51,7 +51,10 @@ struct StepArgs {
   // in 64 bits: these arrays may span more than 4 GiB)
   const void *kv, *kt;
   float ks[NKC];             // per-array scale (binary16 storage), 1 otherwise
-  void *carry[3];            // per axis: NCARRY arrays of slab[ax] elements (carry type)
+  // per axis: the C-PML carries (carry type) of the slab cells, in two groups of records — the
+  // velocity half step's carries 0..3, then the stress half step's 4..7 — each group slab[ax]
+  // records of 4 carries: carry m of slab cell o is element (m/4)*4*slab + 4*o + m%4
+  void *carry[3];
   int64_t slab[3];           // elements per carry array per axis
   const void *prof[3][4];    // per axis: e_int, a_int, e_half, a_half (arithmetic type)
   int nx, ny, nz;
@@ -153,6 +156,8 @@ hipError_t launch_set_counter(int *it, int v, hipStream_t s);
 // multiply a stored array by f (lo: its FP16C residuals or null); carries use prec of the
 // carry type, i.e. pass PREC_FP32 for FP16/FP16C carries
 hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
+// element e of every 4-element record times f[e] (carry groups whose carries scale differently)
+hipError_t launch_rescale4(int prec, void *p, int64_t nrec, const double f[4], hipStream_t s);
 // per-cell packed copies kv (K[0..13]) and kt (K[14..21]) of the 22 coefficient arrays
 hipError_t launch_pack_coef(int prec, const void *soa, void *kv, void *kt, int64_t n, hipStream_t s);
 hipError_t launch_record(int prec, const RecordArgs &a, hipStream_t s);

@@ -448,11 +448,12 @@ int biot_handle::rescale_state(const float old_scale[NFIELD])
   const double rq = (double)old_scale[3] / fscale[3];
   for (int ax = 0; ax < 3 && ok; ++ax) {
     if (slab[ax] == 0) continue;
-    for (int c = 0; c < NCARRY && ok; ++c) {
-      const double f = c < 4 ? rt : (c == 5 ? rq : rv);
-      ok = rescale(prec == BIOT_FP64 ? BIOT_FP64 : prec == BIOT_FP16 ? BIOT_FP16 : BIOT_FP32,
-                   (char *)d_carry[ax] + (size_t)c * slab[ax] * csz, nullptr, slab[ax], f);
-    }
+    // records of carries 0..3 (all in stress units), then 4..7 (velocity units, 5 in q units)
+    const int cp = prec == BIOT_FP64 ? BIOT_FP64 : prec == BIOT_FP16 ? BIOT_FP16 : BIOT_FP32;
+    ok = rescale(cp, d_carry[ax], nullptr, 4 * slab[ax], rt);
+    const double f4[4] = {rv, rq, rv, rv};
+    if (ok && (rv != 1.0 || rq != 1.0))
+      ok = launch_rescale4(cp, (char *)d_carry[ax] + (size_t)4 * slab[ax] * csz, slab[ax], f4, stream) == hipSuccess;
   }
   if (!ok || hipStreamSynchronize(stream) != hipSuccess) {
     err = "rescaling the stored wavefields failed";

@@ -223,18 +223,23 @@ __device__ __forceinline__ typename P::A pml(typename P::SC &cv, typename P::A D
   cv = (SC)(e * (psi + aa * D));
   return D + psi;
 }
-// the four carries of one slab at o, o + sl, o + 2 sl, o + 3 sl
+// the record of four carries of slab cell o in group g (0: velocity half step, 1: stress half
+// step): one vector load and one vector store
 template <class SC>
-__device__ __forceinline__ void carry_load(SC (&cv)[4], const void *base, u32 o, u32 sl)
+__device__ __forceinline__ void carry_load(SC (&cv)[4], const void *base, u32 g, u32 o, u32 sl)
 {
-#pragma unroll
-  for (int m = 0; m < 4; ++m) cv[m] = at<SC>(base, o + (u32)m * sl);
+  typedef SC V4 __attribute__((ext_vector_type(4)));
+  const V4 v = *reinterpret_cast<const V4 *>(reinterpret_cast<const char *>(base) +
+                                             (g * sl + o) * (u32)(4 * sizeof(SC)));
+  cv[0] = v.x, cv[1] = v.y, cv[2] = v.z, cv[3] = v.w;
 }
 template <class SC>
-__device__ __forceinline__ void carry_store(const SC (&cv)[4], void *base, u32 o, u32 sl)
+__device__ __forceinline__ void carry_store(const SC (&cv)[4], void *base, u32 g, u32 o, u32 sl)
 {
-#pragma unroll
-  for (int m = 0; m < 4; ++m) at<SC>(base, o + (u32)m * sl) = cv[m];
+  typedef SC V4 __attribute__((ext_vector_type(4)));
+  V4 v;
+  v.x = cv[0], v.y = cv[1], v.z = cv[2], v.w = cv[3];
+  *reinterpret_cast<V4 *>(reinterpret_cast<char *>(base) + (g * sl + o) * (u32)(4 * sizeof(SC))) = v;
 }
 
 // slab position of index i on an axis with widths L (low) and R (high), or -1
@@ -365,38 +370,38 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     A ezi = 0, azi = 0, ezh = 0, azh = 0;
     if (px >= 0) {
       ox = (u32)px + wxs * ((u32)j + ny * (u32)k);
-      carry_load(cx, a.carry[0], ox, sl0);
+      carry_load(cx, a.carry[0], 0u, ox, sl0);
     }
     if (py >= 0) {
       oy = (u32)i + nx * ((u32)py + wys * (u32)k);
-      carry_load(cy, a.carry[1], oy, sl1);
+      carry_load(cy, a.carry[1], 0u, oy, sl1);
     }
     if (pz >= 0) {
       ezi = prof<A>(a.prof[2][0], k), azi = prof<A>(a.prof[2][1], k);
       ezh = prof<A>(a.prof[2][2], k), azh = prof<A>(a.prof[2][3], k);
       oz = (u32)i + nx * ((u32)j + ny * (u32)pz);
-      carry_load(cz, a.carry[2], oz, sl2);
+      carry_load(cz, a.carry[2], 0u, oz, sl2);
     }
     if (px >= 0) {
       x1 = pml<P>(cx[0], x1, exh, axh);
       s1 = pml<P>(cx[1], s1, exh, axh);
       y2 = pml<P>(cx[2], y2, exi, axi);
       z2 = pml<P>(cx[3], z2, exi, axi);
-      carry_store(cx, a.carry[0], ox, sl0);
+      carry_store(cx, a.carry[0], 0u, ox, sl0);
     }
     if (py >= 0) {
       x2 = pml<P>(cy[0], x2, eyi, ayi);
       y1 = pml<P>(cy[1], y1, eyh, ayh);
       s2 = pml<P>(cy[2], s2, eyh, ayh);
       z3 = pml<P>(cy[3], z3, eyi, ayi);
-      carry_store(cy, a.carry[1], oy, sl1);
+      carry_store(cy, a.carry[1], 0u, oy, sl1);
     }
     if (pz >= 0) {
       x3 = pml<P>(cz[0], x3, ezi, azi);
       y3 = pml<P>(cz[1], y3, ezi, azi);
       z1 = pml<P>(cz[2], z1, ezh, azh);
       s3 = pml<P>(cz[3], s3, ezh, azh);
-      carry_store(cz, a.carry[2], oz, sl2);
+      carry_store(cz, a.carry[2], 0u, oz, sl2);
     }
 
     // relative fluid velocity (kernel.cu:218-220; qz takes W3y as the reference does) and
@@ -512,39 +517,39 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     u32 ox = 0, oy = 0, oz = 0;
     A ezi = 0, azi = 0, ezh = 0, azh = 0;
     if (px >= 0) {
-      ox = 4 * sl0 + (u32)px + wxs * ((u32)j + ny * (u32)k);
-      carry_load(cx, a.carry[0], ox, sl0);
+      ox = (u32)px + wxs * ((u32)j + ny * (u32)k);
+      carry_load(cx, a.carry[0], 1u, ox, sl0);
     }
     if (py >= 0) {
-      oy = 4 * sl1 + (u32)i + nx * ((u32)py + wys * (u32)k);
-      carry_load(cy, a.carry[1], oy, sl1);
+      oy = (u32)i + nx * ((u32)py + wys * (u32)k);
+      carry_load(cy, a.carry[1], 1u, oy, sl1);
     }
     if (pz >= 0) {
       ezi = prof<A>(a.prof[2][0], k), azi = prof<A>(a.prof[2][1], k);
       ezh = prof<A>(a.prof[2][2], k), azh = prof<A>(a.prof[2][3], k);
-      oz = 4 * sl2 + (u32)i + nx * ((u32)j + ny * (u32)pz);
-      carry_load(cz, a.carry[2], oz, sl2);
+      oz = (u32)i + nx * ((u32)j + ny * (u32)pz);
+      carry_load(cz, a.carry[2], 1u, oz, sl2);
     }
     if (px >= 0) {
       uxx = pml<P>(cx[0], uxx, exi, axi);
       wx = pml<P>(cx[1], wx, exi, axi);
       uyx = pml<P>(cx[2], uyx, exh, axh);
       uzx = pml<P>(cx[3], uzx, exh, axh);
-      carry_store(cx, a.carry[0], ox, sl0);
+      carry_store(cx, a.carry[0], 1u, ox, sl0);
     }
     if (py >= 0) {
       uyy = pml<P>(cy[0], uyy, eyi, ayi);
       wy = pml<P>(cy[1], wy, eyi, ayi);
       uxy = pml<P>(cy[2], uxy, eyh, ayh);
       uzy = pml<P>(cy[3], uzy, eyh, ayh);
-      carry_store(cy, a.carry[1], oy, sl1);
+      carry_store(cy, a.carry[1], 1u, oy, sl1);
     }
     if (pz >= 0) {
       uzz = pml<P>(cz[0], uzz, ezi, azi);
       wz = pml<P>(cz[1], wz, ezi, azi);
       uxz = pml<P>(cz[2], uxz, ezh, azh);
       uyz = pml<P>(cz[3], uyz, ezh, azh);
-      carry_store(cz, a.carry[2], oz, sl2);
+      carry_store(cz, a.carry[2], 1u, oz, sl2);
     }
 
     // pore pressure and stresses (kernel.cu:484-490)
@@ -727,6 +732,18 @@ __global__ void k_pack_coef(const S *soa, S *kv, S *kt, int64_t n)
   }
 }
 
+template <typename S>
+__global__ void k_rescale4(S *p, int64_t nrec, double f0, double f1, double f2, double f3)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * nrec; i += stride) {
+    const int e = (int)(i & 3);
+    const double f = e == 0 ? f0 : (e == 1 ? f1 : (e == 2 ? f2 : f3));
+    if constexpr (std::is_same<S, double>::value) p[i] = p[i] * f;
+    else p[i] = (S)((float)p[i] * (float)f);
+  }
+}
+
 template <class P, int ORDER, bool VEL>
 hipError_t launch_step(const StepArgs &in, LaunchShape sh, hipStream_t s)
 {
@@ -901,6 +918,27 @@ hipError_t launch_pack_coef(int prec, const void *soa, void *kv, void *kt, int64
     case PREC_FP64:
       hipLaunchKernelGGL(k_pack_coef<double>, dim3(blocks), dim3(256), 0, s, (const double *)soa, (double *)kv,
                          (double *)kt, n);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rescale4(int prec, void *p, int64_t nrec, const double f[4], hipStream_t s)
+{
+  if (nrec <= 0 || !p) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<int64_t>((4 * nrec + 255) / 256, 65536);
+  switch (prec) {
+    case PREC_FP32:
+    case PREC_FP16C:
+      hipLaunchKernelGGL(k_rescale4<float>, dim3(blocks), dim3(256), 0, s, (float *)p, nrec, f[0], f[1], f[2], f[3]);
+      break;
+    case PREC_FP16:
+      hipLaunchKernelGGL(k_rescale4<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, nrec, f[0], f[1], f[2],
+                         f[3]);
+      break;
+    case PREC_FP64:
+      hipLaunchKernelGGL(k_rescale4<double>, dim3(blocks), dim3(256), 0, s, (double *)p, nrec, f[0], f[1], f[2], f[3]);
       break;
     default: return hipErrorInvalidValue;
   }

@@ -471,25 +471,25 @@ __global__ void __launch_bounds__(256) k_velocity_xv(const StepArgs a)
 
       // C-PML memory (kernel.cu:188-216), same carries and order as the scalar kernel
       if (px[e] >= 0) {
-        const u32 o = (u32)px[e] + wxs * ((u32)j + ny * (u32)k);
+        const u32 o = 4u * ((u32)px[e] + wxs * ((u32)j + ny * (u32)k));  // record of carries 0..3
         x1 = pml<P>(a.carry[0], o, x1, exh[e], axh[e]);
-        s1 = pml<P>(a.carry[0], o + sl0, s1, exh[e], axh[e]);
-        y2 = pml<P>(a.carry[0], o + 2 * sl0, y2, exi[e], axi[e]);
-        z2 = pml<P>(a.carry[0], o + 3 * sl0, z2, exi[e], axi[e]);
+        s1 = pml<P>(a.carry[0], o + 1u, s1, exh[e], axh[e]);
+        y2 = pml<P>(a.carry[0], o + 2u, y2, exi[e], axi[e]);
+        z2 = pml<P>(a.carry[0], o + 3u, z2, exi[e], axi[e]);
       }
       if (py >= 0 && on[e]) {
-        const u32 o = (u32)(i0 + e) + nx * ((u32)py + wys * (u32)k);
+        const u32 o = 4u * ((u32)(i0 + e) + nx * ((u32)py + wys * (u32)k));
         x2 = pml<P>(a.carry[1], o, x2, eyi, ayi);
-        y1 = pml<P>(a.carry[1], o + sl1, y1, eyh, ayh);
-        s2 = pml<P>(a.carry[1], o + 2 * sl1, s2, eyh, ayh);
-        z3 = pml<P>(a.carry[1], o + 3 * sl1, z3, eyi, ayi);
+        y1 = pml<P>(a.carry[1], o + 1u, y1, eyh, ayh);
+        s2 = pml<P>(a.carry[1], o + 2u, s2, eyh, ayh);
+        z3 = pml<P>(a.carry[1], o + 3u, z3, eyi, ayi);
       }
       if (pz >= 0 && on[e]) {
-        const u32 o = (u32)(i0 + e) + nx * ((u32)j + ny * (u32)pz);
+        const u32 o = 4u * ((u32)(i0 + e) + nx * ((u32)j + ny * (u32)pz));
         x3 = pml<P>(a.carry[2], o, x3, ezi, azi);
-        y3 = pml<P>(a.carry[2], o + sl2, y3, ezi, azi);
-        z1 = pml<P>(a.carry[2], o + 2 * sl2, z1, ezh, azh);
-        s3 = pml<P>(a.carry[2], o + 3 * sl2, s3, ezh, azh);
+        y3 = pml<P>(a.carry[2], o + 1u, y3, ezi, azi);
+        z1 = pml<P>(a.carry[2], o + 2u, z1, ezh, azh);
+        s3 = pml<P>(a.carry[2], o + 3u, s3, ezh, azh);
       }
       // relative fluid velocity (kernel.cu:218-220; qz takes W3y) and solid velocity
       // with the stored new q (kernel.cu:223-225)
@@ -662,25 +662,25 @@ __global__ void __launch_bounds__(256) k_stress_xv(const StepArgs a)
 
       // C-PML memory (kernel.cu:449-482): carries 4..7 of each axis
       if (px[e] >= 0) {
-        const u32 o = 4 * sl0 + (u32)px[e] + wxs * ((u32)j + ny * (u32)k);
+        const u32 o = 4 * sl0 + 4u * ((u32)px[e] + wxs * ((u32)j + ny * (u32)k));  // carries 4..7
         uxx = pml<P>(a.carry[0], o, uxx, exi[e], axi[e]);
-        wx = pml<P>(a.carry[0], o + sl0, wx, exi[e], axi[e]);
-        uyx = pml<P>(a.carry[0], o + 2 * sl0, uyx, exh[e], axh[e]);
-        uzx = pml<P>(a.carry[0], o + 3 * sl0, uzx, exh[e], axh[e]);
+        wx = pml<P>(a.carry[0], o + 1u, wx, exi[e], axi[e]);
+        uyx = pml<P>(a.carry[0], o + 2u, uyx, exh[e], axh[e]);
+        uzx = pml<P>(a.carry[0], o + 3u, uzx, exh[e], axh[e]);
       }
       if (py >= 0 && on[e]) {
-        const u32 o = 4 * sl1 + (u32)(i0 + e) + nx * ((u32)py + wys * (u32)k);
+        const u32 o = 4 * sl1 + 4u * ((u32)(i0 + e) + nx * ((u32)py + wys * (u32)k));
         uyy = pml<P>(a.carry[1], o, uyy, eyi, ayi);
-        wy = pml<P>(a.carry[1], o + sl1, wy, eyi, ayi);
-        uxy = pml<P>(a.carry[1], o + 2 * sl1, uxy, eyh, ayh);
-        uzy = pml<P>(a.carry[1], o + 3 * sl1, uzy, eyh, ayh);
+        wy = pml<P>(a.carry[1], o + 1u, wy, eyi, ayi);
+        uxy = pml<P>(a.carry[1], o + 2u, uxy, eyh, ayh);
+        uzy = pml<P>(a.carry[1], o + 3u, uzy, eyh, ayh);
       }
       if (pz >= 0 && on[e]) {
-        const u32 o = 4 * sl2 + (u32)(i0 + e) + nx * ((u32)j + ny * (u32)pz);
+        const u32 o = 4 * sl2 + 4u * ((u32)(i0 + e) + nx * ((u32)j + ny * (u32)pz));
         uzz = pml<P>(a.carry[2], o, uzz, ezi, azi);
-        wz = pml<P>(a.carry[2], o + sl2, wz, ezi, azi);
-        uxz = pml<P>(a.carry[2], o + 2 * sl2, uxz, ezh, azh);
-        uyz = pml<P>(a.carry[2], o + 3 * sl2, uyz, ezh, azh);
+        wz = pml<P>(a.carry[2], o + 1u, wz, ezi, azi);
+        uxz = pml<P>(a.carry[2], o + 2u, uxz, ezh, azh);
+        uyz = pml<P>(a.carry[2], o + 3u, uyz, ezh, azh);
       }
       // pore pressure and stresses (kernel.cu:484-490)
       const A divw = wx + wy + wz;
