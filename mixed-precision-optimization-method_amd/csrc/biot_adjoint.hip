@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include "biot_adjoint.hpp"
+#include "biot_packed.hpp"
 
 namespace biot {
 namespace {
@@ -158,6 +159,50 @@ __device__ __forceinline__ typename AdjT<S>::A kcoef(const AT &a, int kidx, u32 
   return v;
 }
 
+// the N coefficients K[k0..k0+N) of cell c from the packed copy (storage units, times the
+// per-array scale for 16-bit storage)
+template <typename S, int N, class AT>
+__device__ __forceinline__ void kblock(const AT &a, const void *base, u32 c, int k0,
+                                       typename AdjT<S>::A (&out)[N])
+{
+  using A = typename AdjT<S>::A;
+  uint32_t d[packed::block_dwords<S, N>()];
+  packed::load<S, N, true>(base, (uint64_t)a.nx * a.ny * a.nz, c, d);
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    if constexpr (AdjT<S>::scaled) out[e] = (A)packed::value<S>(d, e) * (A)a.ks[k0 + e];
+    else out[e] = (A)packed::value<S>(d, e);
+  }
+}
+// the N gradient sums of K[k0..k0+N) at cell c: float partial sums of the packed array p on
+// FP32/FP16 handles, the FP64 sums themselves on FP64 handles
+template <typename A, int N, class AT>
+__device__ __forceinline__ void gload(const AT &a, const void *p, u32 c, int k0, double (&g)[N])
+{
+  if constexpr (std::is_same<A, double>::value) {
+#pragma unroll
+    for (int e = 0; e < N; ++e) g[e] = at<double>(a.gacc[k0 + e], c);
+  } else {
+    uint32_t d[packed::block_dwords<float, N>()];
+    packed::load<float, N, false>(p, (uint64_t)a.nx * a.ny * a.nz, c, d);
+#pragma unroll
+    for (int e = 0; e < N; ++e) g[e] = (double)packed::value<float>(d, e);
+  }
+}
+template <typename A, int N, class AT>
+__device__ __forceinline__ void gstore(const AT &a, void *p, u32 c, int k0, const double (&g)[N])
+{
+  if constexpr (std::is_same<A, double>::value) {
+#pragma unroll
+    for (int e = 0; e < N; ++e) at<double>(a.gacc[k0 + e], c) = g[e];
+  } else {
+    uint32_t d[N];
+#pragma unroll
+    for (int e = 0; e < N; ++e) d[e] = __builtin_bit_cast(uint32_t, (float)g[e]);
+    packed::store<float, N>(p, (uint64_t)a.nx * a.ny * a.nz, c, d);
+  }
+}
+
 // Loads of one derivative-table entry: the forward derivative E (fields of state qf) and, in
 // the entry's absorbing slab, the profile values, the forward carry (state qc) and the carry
 // adjoint.  Every kernel below issues all of its loads before its first store: the compiler
@@ -214,17 +259,16 @@ __device__ __forceinline__ void adj_stress_local(const AT &a, int q, u32 t)
   const A ltxx = at<A>(a.lam[6], c), ltyy = at<A>(a.lam[7], c), ltzz = at<A>(a.lam[8], c);
   const A ltxy = at<A>(a.lam[9], c), ltxz = at<A>(a.lam[10], c), ltyz = at<A>(a.lam[11], c);
   const A lp = at<A>(a.lam[12], c);
-  const A S1 = kcoef<S>(a, K_S1, c), S2 = kcoef<S>(a, K_S2, c), S3 = kcoef<S>(a, K_S3, c);
-  const A P1 = kcoef<S>(a, K_P1, c), P2 = kcoef<S>(a, K_P2, c);
-  const A Sxy = kcoef<S>(a, K_SXY, c), Sxz = kcoef<S>(a, K_SXZ, c), Syz = kcoef<S>(a, K_SYZ, c);
+  A kt[8];  // K[14..21]: P1 P2 S1 S2 S3 Sxy Sxz Syz
+  kblock<S, 8>(a, a.kt, c, K_P1, kt);
+  const A P1 = kt[0], P2 = kt[1], S1 = kt[2], S2 = kt[3], S3 = kt[4];
+  const A Sxy = kt[5], Sxz = kt[6], Syz = kt[7];
   PmlIn<A> in[12];
 #pragma unroll
   for (int d = 0; d < 12; ++d)  // velocities of state q+1 (after the velocity half step), carries of state q
     pml_load<S, ORDER>(a, TDER[d], ci, q + 1, q, in[d]);
-  constexpr int KG[8] = {K_P1, K_P2, K_S1, K_S2, K_S3, K_SXY, K_SXZ, K_SYZ};
-  double g[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) g[m] = (double)at<A>(a.gacc[KG[m]], c);
+  double g[8];  // sums of K[14..21]
+  gload<A, 8>(a, a.gpt, c, K_P1, g);
 
   A lE[12];
   lE[0] = S2 * ltxx + S1 * (ltyy + ltzz) - P1 * lp;
@@ -252,8 +296,7 @@ __device__ __forceinline__ void adj_stress_local(const AT &a, int q, u32 t)
     at<A>(a.G[d], c) = G[d];
     pml_store(a, TDER[d], in[d]);
   }
-#pragma unroll
-  for (int m = 0; m < 8; ++m) at<A>(a.gacc[KG[m]], c) = (A)g[m];
+  gstore<A, 8>(a, a.gpt, c, K_P1, g);
 }
 
 // lambda(v', q') += transposed differences of the stress-side derivative adjoints
@@ -295,27 +338,30 @@ __device__ __forceinline__ void adj_velocity_local(const AT &a, int q, u32 t)
   constexpr int kW3[3] = {K_W3X, K_W3Y, K_W3Y};  // qz takes W3y (kernel.cu:220)
   // loads: adjoint state, coefficients, forward q of states q and q+1, derivative-table
   // entries, gradient sums
+  A kv[14];  // K[0..13]: U1 xyz, U2 xyz, W1 xyz, W2 xyz, W3x, W3y
+  kblock<S, 14>(a, a.kv, c, 0, kv);
   A lv[3], lq[3], U1[3], U2[3], W1[3], W2[3], W3[3];
   double q0[3], q1[3], gU1[3], gU2[3], gW1[3], gW2[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     lv[m] = at<A>(a.lam[m], c);
     lq[m] = at<A>(a.lam[3 + m], c);
-    U1[m] = kcoef<S>(a, kU1[m], c), U2[m] = kcoef<S>(a, kU2[m], c);
-    W1[m] = kcoef<S>(a, kW1[m], c), W2[m] = kcoef<S>(a, kW2[m], c), W3[m] = kcoef<S>(a, kW3[m], c);
+    U1[m] = kv[kU1[m]], U2[m] = kv[kU2[m]];
+    W1[m] = kv[kW1[m]], W2[m] = kv[kW2[m]], W3[m] = kv[kW3[m]];
     q0[m] = (double)at<S>(fw_field(a, q, 3 + m), c);
     q1[m] = (double)at<S>(fw_field(a, q + 1, 3 + m), c);
   }
   PmlIn<A> in[12];
 #pragma unroll
   for (int d = 0; d < 12; ++d) pml_load<S, ORDER>(a, VDER[d], ci, q, q, in[d]);
-  auto gl = [&](int kk) { return (double)at<A>(a.gacc[kk], c); };
+  double gv[14];  // sums of K[0..13]
+  gload<A, 14>(a, a.gpv, c, 0, gv);
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    gU1[m] = gl(kU1[m]), gU2[m] = gl(kU2[m]);
-    gW1[m] = gl(kW1[m]), gW2[m] = gl(kW2[m]);
+    gU1[m] = gv[kU1[m]], gU2[m] = gv[kU2[m]];
+    gW1[m] = gv[kW1[m]], gW2[m] = gv[kW2[m]];
   }
-  double gW3x = gl(K_W3X), gW3y = gl(K_W3Y);
+  double gW3x = gv[K_W3X], gW3y = gv[K_W3Y];
 
   // per component: lambda_qe = lambda_q' - U2 lambda_v'; lambda_S = U1 lambda_v' - W2 lambda_qe;
   // lambda_Ds = -W3 lambda_qe (qz takes W3y, kernel.cu:220)
@@ -355,14 +401,14 @@ __device__ __forceinline__ void adj_velocity_local(const AT &a, int q, u32 t)
     at<A>(a.G[d], c) = G[d];
     pml_store(a, VDER[d], in[d]);
   }
-  auto gs = [&](int kk, double v) { at<A>(a.gacc[kk], c) = (A)v; };
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    gs(kU1[m], gU1[m]), gs(kU2[m], gU2[m]);
-    gs(kW1[m], gW1[m]), gs(kW2[m], gW2[m]);
+    gv[kU1[m]] = gU1[m], gv[kU2[m]] = gU2[m];
+    gv[kW1[m]] = gW1[m], gv[kW2[m]] = gW2[m];
   }
-  gs(K_W3X, gW3x);
-  gs(K_W3Y, gW3y);
+  gv[K_W3X] = gW3x;
+  gv[K_W3Y] = gW3y;
+  gstore<A, 14>(a, a.gpv, c, 0, gv);
 }
 
 // lambda(tau, p) += transposed differences of the velocity-side derivative adjoints
@@ -399,17 +445,19 @@ __device__ __forceinline__ void adj_flush(const AT &a, u32 t)
   CellIdx ci;
   if (t >= a.nupd || !cell_of(a, h, t, ci)) return;
   const u32 c = ci.c;
-  float part[NKC];
-#pragma unroll
-  for (int k = 0; k < NKC; ++k) part[k] = at<float>(a.gacc[k], c);
+  const uint64_t n = (uint64_t)a.nx * a.ny * a.nz;
+  uint32_t dv[14], dt[8];
+  packed::load<float, 14, false>(a.gpv, n, c, dv);
+  packed::load<float, 8, false>(a.gpt, n, c, dt);
   double tot[NKC];
 #pragma unroll
   for (int k = 0; k < NKC; ++k) tot[k] = a.grad[k][c];
 #pragma unroll
-  for (int k = 0; k < NKC; ++k) {
-    a.grad[k][c] = tot[k] + (double)part[k];
-    at<float>(a.gacc[k], c) = 0.0f;
-  }
+  for (int k = 0; k < NKC; ++k)
+    a.grad[k][c] = tot[k] + (double)(k < 14 ? packed::value<float>(dv, k) : packed::value<float>(dt, k - 14));
+  const uint32_t zero[14] = {};
+  packed::store<float, 14>(a.gpv, n, c, zero);
+  packed::store<float, 8>(a.gpt, n, c, zero);
 }
 
 template <typename S, int ORDER, int WHICH, class AT>

@@ -24,7 +24,12 @@ struct AdjArgs {
   // FP16 handles; launch_adjoint_flush adds them into grad and clears them), grad itself on
   // FP64 handles
   void *gacc[NKC];
+  // FP32/FP16 handles: the same partial sums packed per cell (float; gpv: K[0..13] of the
+  // velocity half step, gpt: K[14..21] of the stress half step, in the 16-byte group layout
+  // of biot_packed.hpp); null on FP64 handles
+  void *gpv, *gpt;
   const void *k[NKC];        // kernel coefficients (storage type)
+  const void *kv, *kt;       // the coefficients packed per cell (storage type, same layout)
   float ks[NKC];             // per-array coefficient scale (16-bit storage), else 1
   const void *prof[3][4];    // e_int, a_int, e_half, a_half per axis (arithmetic type)
   int nx, ny, nz;
@@ -49,7 +54,8 @@ struct InjectArgs {
 // step q; prec is the storage precision (FP32, FP64 or FP16)
 hipError_t launch_adjoint_step(int prec, int order, const AdjArgs &a, int q, hipStream_t s);
 hipError_t launch_adjoint_inject(int prec, const InjectArgs &a, int it, hipStream_t s);
-// grad += gacc, gacc = 0 over the update box (no-op on FP64 handles, which sum into grad)
+// grad += packed partial sums, partial sums = 0 over the update box (no-op on FP64 handles,
+// which sum into grad)
 hipError_t launch_adjoint_flush(int prec, int order, const AdjArgs &a, hipStream_t s);
 // J += 1/2 sum (d - obs)^2 (FP64), res = d - obs (res may be null); d is double when
 // d_double, else float

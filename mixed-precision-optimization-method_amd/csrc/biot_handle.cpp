@@ -125,6 +125,10 @@ void biot_handle::adj_args(AdjArgs &A) const
     A.k[k] = (const char *)d_coef + (size_t)k * n * esz;
     A.ks[k] = ks[k];
   }
+  A.gpv = prec == BIOT_FP64 ? nullptr : (void *)d_gpart;
+  A.gpt = prec == BIOT_FP64 ? nullptr : (void *)(d_gpart + (size_t)14 * n);
+  A.kv = d_kpack;
+  A.kt = (const char *)d_kpack + (size_t)14 * n * esz;
   for (int ax = 0; ax < 3; ++ax) {
     for (int q = 0; q < 4; ++q) A.prof[ax][q] = d_prof[ax][q];
     A.L[ax] = L[ax];

@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "biot_device.hpp"
+#include "biot_packed.hpp"
 
 namespace biot {
 namespace {
@@ -157,57 +158,20 @@ __device__ __forceinline__ typename P::A dbw(const AT &a, u32 c, u32 s, typename
   }
 }
 
-// The N coefficients of cell c from a packed block.  The block stores the values of a cell in
-// 16-byte groups, group-major: group g (values g*G .. g*G+G-1, G = 16 / sizeof(S)) is an array
-// of n 16-byte records at base + g*n*16, and the last N mod G values form a tail array of
-// (N mod G)*sizeof(S)-byte records after the full groups.  Every load of a wavefront is thus
-// one contiguous run (a dwordx4, or the tail's dwordx3/x2, per lane); addresses are 64-bit
-// since a block may span more than 4 GiB.  Each value is multiplied by its array's scale.
-typedef uint32_t dw4_t __attribute__((ext_vector_type(4), aligned(4)));
-typedef uint32_t dw3_t __attribute__((ext_vector_type(3), aligned(4)));
-typedef uint32_t dw2_t __attribute__((ext_vector_type(2), aligned(4)));
-template <int ND>
-__device__ __forceinline__ void ld_rec(const char *p, uint32_t *d)
-{
-  if constexpr (ND == 4) {
-    const dw4_t v = __builtin_nontemporal_load(reinterpret_cast<const dw4_t *>(p));
-    d[0] = v.x, d[1] = v.y, d[2] = v.z, d[3] = v.w;
-  } else if constexpr (ND == 3) {
-    const dw3_t v = __builtin_nontemporal_load(reinterpret_cast<const dw3_t *>(p));
-    d[0] = v.x, d[1] = v.y, d[2] = v.z;
-  } else if constexpr (ND == 2) {
-    const dw2_t v = __builtin_nontemporal_load(reinterpret_cast<const dw2_t *>(p));
-    d[0] = v.x, d[1] = v.y;
-  } else if constexpr (ND == 1) {
-    d[0] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
-  }
-}
+// The N coefficients of cell c from a packed block (biot_packed.hpp layout; nontemporal:
+// read once per half step), each times its array's scale for 16-bit storage.
 template <class P, int N, class AT>
 __device__ __forceinline__ void coef_block(const AT &a, const void *base, u32 c, int k0,
                                            typename P::A (&out)[N])
 {
   using S = typename P::S;
   using A = typename P::A;
-  constexpr int G = 16 / (int)sizeof(S);
-  constexpr int NG = N / G, TAIL = N % G;
-  constexpr int TB = TAIL * (int)sizeof(S);
-  static_assert(TB % 4 == 0, "tail records must be whole dwords");
-  const char *b = reinterpret_cast<const char *>(base);
-  const uint64_t n16 = (uint64_t)a.nx * a.ny * a.nz * 16u;
-  uint32_t d[(N * (int)sizeof(S) + 3) / 4];
-#pragma unroll
-  for (int g = 0; g < NG; ++g) ld_rec<4>(b + g * n16 + (uint64_t)c * 16u, d + 4 * g);
-  if constexpr (TAIL > 0) ld_rec<TB / 4>(b + NG * n16 + (uint64_t)c * TB, d + 4 * NG);
+  uint32_t d[packed::block_dwords<S, N>()];
+  packed::load<S, N, true>(base, (uint64_t)a.nx * a.ny * a.nz, c, d);
 #pragma unroll
   for (int e = 0; e < N; ++e) {
-    if constexpr (std::is_same<S, _Float16>::value) {
-      const uint16_t bits = (uint16_t)(d[e >> 1] >> (16 * (e & 1)));
-      out[e] = (A)__builtin_bit_cast(_Float16, bits) * a.ks[k0 + e];
-    } else if constexpr (std::is_same<S, float>::value) {
-      out[e] = __builtin_bit_cast(float, d[e]);
-    } else {
-      out[e] = __builtin_bit_cast(double, (uint64_t)d[2 * e] | ((uint64_t)d[2 * e + 1] << 32));
-    }
+    if constexpr (P::scaled) out[e] = (A)packed::value<S>(d, e) * a.ks[k0 + e];
+    else out[e] = (A)packed::value<S>(d, e);
   }
 }
 
