@@ -150,15 +150,6 @@ __device__ __forceinline__ A profv(const AT &a, int ax, int q, const CellIdx &ci
   return at<A>(a.prof[ax][q], (u32)pos);
 }
 
-template <typename S, class AT>
-__device__ __forceinline__ typename AdjT<S>::A kcoef(const AT &a, int kidx, u32 c)
-{
-  using A = typename AdjT<S>::A;
-  const A v = (A)at<S>(a.k[kidx], c);
-  if constexpr (AdjT<S>::scaled) return v * (A)a.ks[kidx];
-  return v;
-}
-
 // the N coefficients K[k0..k0+N) of cell c from the packed copy (storage units, times the
 // per-array scale for 16-bit storage)
 template <typename S, int N, class AT>

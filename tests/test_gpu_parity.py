@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from biot_amd import BiotError, configs, media
+from biot_amd import BiotError, configs, media, ricker
 from biot_amd.solver import Solver
 from conftest import max_comp_rel_l2, rel_l2
 
@@ -180,6 +180,45 @@ def test_full_size_random_state_matches_oracle(precision, mode, tol):
     print(f"  worst field rel L2 = {worst:.3e}")
     o.close()
     s.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp16", 3e-2), ("fp16c", 1e-3)])
+def test_storage_scale_change_mid_run(precision, tol):
+    """A new source amplitude changes the 16-bit storage scales, and the handle re-expresses
+    every stored field and C-PML carry in the new units.  A source that is silent after step
+    80 is replaced at step 120 (after the wave has entered the absorbing layers) by 16x itself:
+    the physics is unchanged, so the run with the replacement must keep the accuracy of the
+    run without it against the FP64 run (a wrongly scaled carry would not: the late traces
+    carry the absorbing layers' response)."""
+    print(f"scale change {precision}", flush=True)
+    n, npml, nt = 48, 8, 240
+    m = media.homogeneous((n, n, n))
+    c = n // 2
+    src = c + c * n + c * n * n
+    w = np.zeros(nt, np.float32)
+    w[:80] = ricker(40.0, 4e-4, 80)
+    rec = np.array([c + c * n + (npml + 2) * n * n, 3 + c * n + c * n * n, c + 4 * n + c * n * n], np.int64)
+    out = {}
+    for name, prec, swap in (("ref", "fp64", False), ("plain", precision, False), ("swap", precision, True)):
+        s = Solver(n, n, n, 5.0, 4e-4, order=4, precision=prec, npml=npml)
+        s.set_medium(m)
+        s.set_source(src, w, ("txx", "tyy", "tzz", "p"))
+        s.set_receivers(rec, ("vx", "vz", "p"), nt)
+        if swap:
+            s.run(120)
+            s.set_source(src, w * 16, ("txx", "tyy", "tzz", "p"))
+            s.run(nt - 120)
+        else:
+            s.run(nt)
+        out[name] = s.traces().astype(np.float64)
+        s.close()
+    ref = out["ref"]
+    e_plain = max(rel_l2(out["plain"][q], ref[q]) for q in range(3))
+    e_swap = max(rel_l2(out["swap"][q], ref[q]) for q in range(3))
+    e_late = max(rel_l2(out["swap"][q][150:], ref[q][150:]) for q in range(3))
+    print(f"  vs fp64: without the change {e_plain:.2e}, with it {e_swap:.2e} (after step 150 {e_late:.2e})")
+    assert np.abs(ref[:, 150:]).max() > 0
+    assert e_swap <= max(1.5 * e_plain, tol) and e_late <= 2 * tol
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16", "fp64"])
