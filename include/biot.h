@@ -174,6 +174,13 @@ int biot_write_binary(const char *path, const float *data, int64_t count);
  * the loop continues; returns when every copy has landed.  Replaces the reference's
  * per-step full-volume copy to the host (kernel.cu:2197-2211). */
 int biot_run_stream_traces(biot_handle *h, int32_t nsteps, int32_t chunk, float *out);
+/* The same loop with xz snapshots as well: after step it0 + snap_steps[s] (it0 the step at the
+ * call; snap_steps ascending within [0, nsteps)) the y = j slice of `field` is written, in
+ * physical units, into snaps + s*nz*nx (row k, column i) and copied to the host on the side
+ * stream while the loop continues (the reference writes these slices at it = 1000, 2000, 3000,
+ * kernel.cu:2247-2385).  traces may be NULL (no trace copies). */
+int biot_run_stream(biot_handle *h, int32_t nsteps, int32_t chunk, float *traces, int32_t field, int32_t j,
+                    const int32_t *snap_steps, int32_t nsnap, float *snaps);
 
 /* ---- inversion parameterization and the adjoint (no counterpart in the reference) ---- */
 /* Per-cell inversion parameters (nx*ny*nz doubles each).  lambda (drained Lame), mu, alpha,

@@ -141,6 +141,7 @@ def lib():
         "biot_write_text_xz": ([C.c_char_p, FP, i32, i32, i32], C.c_int),
         "biot_write_binary": ([C.c_char_p, FP, i64], C.c_int),
         "biot_run_stream_traces": ([H, i32, i32, FP], C.c_int),
+        "biot_run_stream": ([H, i32, i32, FP, i32, i32, C.POINTER(C.c_int32), i32, FP], C.c_int),
         "biot_forward_checkpointed": ([H, i32, i32], C.c_int),
         "biot_misfit_l2": ([H, FP, FP, _DP], C.c_int),
         "biot_adjoint": ([H, FP, C.POINTER(_DP), C.POINTER(_DP)], C.c_int),
@@ -185,7 +186,8 @@ def exported_symbols():
             "biot_read_traces", "biot_read_traces_f64", "biot_read_field",
             "biot_read_field_f64", "biot_write_field_f64", "biot_snapshot_xz",
             "biot_get_field_scales", "biot_device_bytes", "biot_write_text_2d",
-            "biot_write_text_xz", "biot_write_binary", "biot_run_stream_traces", "biot_set_params",
+            "biot_write_text_xz", "biot_write_binary", "biot_run_stream_traces", "biot_run_stream",
+            "biot_set_params",
             "biot_set_medium_device", "biot_kernel_coefficients", "biot_read_kernel_coefficients",
             "biot_forward_checkpointed", "biot_misfit_l2", "biot_adjoint",
             "biot_params_coefficients", "biot_params_chain", "biot_batch_create",

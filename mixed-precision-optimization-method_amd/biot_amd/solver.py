@@ -211,6 +211,22 @@ class Solver:
                     "biot_run_stream_traces")
         return out
 
+    def run_stream(self, nsteps, chunk=100, snap_field="txx", snap_j=None, snap_steps=(),
+                   traces=True):
+        """nsteps like run(), with the traces (if traces) and xz snapshots of snap_field at
+        y = snap_j (default ny/2) after the listed steps (counted from the current step) copied
+        to the host on a side stream; returns (traces or None, snapshots (nsnap, nz, nx))."""
+        steps = np.ascontiguousarray(np.asarray(snap_steps, np.int32))
+        snaps = np.zeros((steps.size, self.nz, self.nx), np.float32)
+        tr = np.zeros((self.ncomp, self.nt_max, self.nrec), np.float32) if traces else None
+        f = FIELD_INDEX[snap_field] if isinstance(snap_field, str) else int(snap_field)
+        j = self.ny // 2 if snap_j is None else int(snap_j)
+        rc = lib().biot_run_stream(self._h, int(nsteps), int(chunk), fptr(tr) if tr is not None else None,
+                                   f, j, steps.ctypes.data_as(C.POINTER(C.c_int32)), steps.size,
+                                   fptr(snaps))
+        self._check(rc, "biot_run_stream")
+        return tr, snaps
+
     def write_traces_text(self, path, comp=0, traces=None):
         """One recorded component in the reference's trace-file layout (sisx*.dat,
         kernel.cu:2557-2571): one row per receiver, nt_max columns."""

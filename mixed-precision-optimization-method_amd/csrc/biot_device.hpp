@@ -158,6 +158,9 @@ hipError_t launch_set_counter(int *it, int v, hipStream_t s);
 hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
 // element e of every 4-element record times f[e] (carry groups whose carries scale differently)
 hipError_t launch_rescale4(int prec, void *p, int64_t nrec, const double f[4], hipStream_t s);
+// the xz slice y = j of one field as float values (stored value times scale): out[k*nx + i]
+hipError_t launch_slice_xz(int prec, const void *f, const int8_t *lo, bool comp, double scale, int j, int nx,
+                           int ny, int nz, float *out, hipStream_t s);
 // per-cell packed copies kv (K[0..13]) and kt (K[14..21]) of the 22 coefficient arrays
 hipError_t launch_pack_coef(int prec, const void *soa, void *kv, void *kt, int64_t n, hipStream_t s);
 hipError_t launch_record(int prec, const RecordArgs &a, hipStream_t s);

@@ -672,6 +672,18 @@ __global__ void k_rescale(S *p, int8_t *lo, int64_t n, double f)
   }
 }
 
+// xz slice of a field in physical units; the product of the stored value and the scale is
+// formed in double and rounded once, as the host read-back does
+template <class P>
+__global__ void k_slice_xz(const void *f, const int8_t *lo, bool comp, double scale, int j, int nx, int ny,
+                           float *out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (i >= nx) return;
+  const u32 c = (u32)i + (u32)nx * ((u32)j + (u32)ny * (u32)k);
+  out[(int64_t)k * nx + i] = (float)((double)load_raw<P>(f, lo, comp, c) * scale);
+}
+
 // packing of the coefficient arrays into the velocity block (K[0..13]) and the stress block
 // (K[14..21]), each in the group-major 16-byte layout coef_block reads
 template <typename S>
@@ -860,6 +872,21 @@ hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hi
     case PREC_FP64:
       hipLaunchKernelGGL(k_rescale<double>, dim3(blocks), dim3(256), 0, s, (double *)p, lo, n, f);
       break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_slice_xz(int prec, const void *f, const int8_t *lo, bool comp, double scale, int j, int nx,
+                           int ny, int nz, float *out, hipStream_t s)
+{
+  if (nx <= 0 || nz <= 0) return hipSuccess;
+  dim3 block(256), grid((unsigned)((nx + 255) / 256), (unsigned)nz);
+  switch (prec) {
+    case PREC_FP32: hipLaunchKernelGGL(k_slice_xz<P32>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
+    case PREC_FP16: hipLaunchKernelGGL(k_slice_xz<P16>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
+    case PREC_FP16C: hipLaunchKernelGGL(k_slice_xz<P16C>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
+    case PREC_FP64: hipLaunchKernelGGL(k_slice_xz<P64>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -1,10 +1,12 @@
-"""GPU check of the streamed trace copy (biot_run_stream_traces): the traces copied to the
-host on a side stream while the loop runs equal the ones read after a plain run, bit for
-bit, for chunk sizes that do and do not divide the step count, and across two calls."""
+"""GPU check of the streamed copies (biot_run_stream_traces, biot_run_stream): the traces and
+xz snapshots copied to the host on a side stream while the loop runs equal the ones read after
+a plain run, bit for bit, for chunk sizes that do and do not divide the step count, for
+snapshot steps inside and at the ends of chunks, and across two calls."""
 import numpy as np
 import pytest
 
 from biot_amd import configs
+from biot_amd.solver import BiotError
 
 pytestmark = pytest.mark.gpu
 
@@ -23,6 +25,51 @@ def test_streamed_traces_equal_plain_run(chunk):
     assert np.array_equal(a[:, :100], ref[:, :100])
     assert not a[:, 100:].any()
     assert np.array_equal(b[:, 100:], ref[:, 100:])
+
+
+@pytest.mark.parametrize("precision,chunk", [("fp32", 7), ("fp32", 64), ("fp16", 50), ("fp16c", 500)])
+def test_streamed_snapshots_equal_plain_run(precision, chunk):
+    print(f"stream snapshots {precision} chunk {chunk}", flush=True)
+    cfg = configs.fwd64(order=4, nt=150)
+    steps = [0, 9, 49, 63, 149]
+    s = configs.make_solver(cfg, precision=precision)
+    ref_snaps, done = [], 0
+    for t in steps:                     # plain run, read after each listed step
+        s.run(t + 1 - done)
+        done = t + 1
+        ref_snaps.append(s.snapshot_xz("tzz", 20))
+    ref_tr = s.traces()
+    s.reset()
+    tr, snaps = s.run_stream(150, chunk, snap_field="tzz", snap_j=20, snap_steps=steps)
+    assert s.step() == 150
+    assert np.array_equal(tr, ref_tr)
+    assert snaps.shape == (len(steps), s.nz, s.nx)
+    for a, b in zip(snaps, ref_snaps):
+        assert np.array_equal(a, b)
+    assert np.abs(snaps[-1]).max() > 0
+    # a second call counts its snapshot steps from its own first step; no traces asked for
+    s.reset()
+    s.run(100)
+    ref = s.snapshot_xz("vx", 32)
+    s.reset()
+    s.run(40)
+    tr2, snaps2 = s.run_stream(60, chunk, snap_field="vx", snap_j=32, snap_steps=[59], traces=False)
+    s.close()
+    assert tr2 is None
+    assert np.array_equal(snaps2[0], ref)
+
+
+def test_streamed_snapshot_arguments():
+    cfg = configs.fwd64(order=2, nt=20)
+    s = configs.make_solver(cfg, precision="fp32")
+    for kw in (dict(snap_steps=[5, 5]), dict(snap_steps=[7, 3]), dict(snap_steps=[10]),
+               dict(snap_steps=[-1]), dict(snap_steps=[2], snap_j=64), dict(snap_steps=[2], snap_field=13)):
+        with pytest.raises(BiotError):
+            s.run_stream(10, 4, **kw)
+        assert s.step() == 0            # rejected before any step ran
+    tr, snaps = s.run_stream(10, 4)     # no snapshots
+    assert snaps.shape == (0, s.nz, s.nx) and s.step() == 10
+    s.close()
 
 
 def test_text_files_of_a_run(tmp_path):
