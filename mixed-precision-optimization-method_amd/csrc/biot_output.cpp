@@ -101,6 +101,7 @@ extern "C" int biot_run_stream(biot_handle *h, int32_t nsteps, int32_t chunk, fl
   }
   const int it0 = h->it;
   const int last = std::min(it0 + nsteps, h->nt_max);  // steps beyond nt_max record nothing
+  chunk = std::min(chunk, std::max(nsteps, 1));       // the staging buffers hold one chunk
   const size_t row = (size_t)h->nrec * sizeof(float);
   const size_t stage_bytes = traces ? (size_t)h->ncomp * chunk * row : 0;
   const size_t slice = (size_t)h->g.nx * h->g.nz;
