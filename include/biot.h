@@ -44,8 +44,8 @@ enum biot_status {
 
 enum biot_precision {
   BIOT_FP32 = 0,  /* float storage, float arithmetic: the drop-in path */
-  BIOT_FP16 = 1,  /* IEEE binary16 wavefields and coefficients with per-field scaling, float
-                     C-PML memory, float arithmetic */
+  BIOT_FP16 = 1,  /* IEEE binary16 wavefields, coefficients and C-PML memory with per-field
+                     scaling, float arithmetic */
   BIOT_FP64 = 2,  /* double storage and arithmetic (accuracy reference on the GPU) */
   BIOT_FP16C = 3  /* as BIOT_FP16, plus an 8-bit residual per value of v, tau and p
                      (value = h + r * ulp(h)/256): compensated half precision */

@@ -18,7 +18,7 @@
 //
 // Storage modes (enum biot_precision):
 //   FP32  fields, coefficients, carries float; arithmetic float.
-//   FP16  fields and coefficients binary16 (scaled), carries float; arithmetic float.
+//   FP16  fields, coefficients and carries binary16 (scaled); arithmetic float.
 //   FP16C as FP16, plus an int8 residual r per value of v, tau and p: the value is
 //         h + r * ulp(h) / 256 (h the binary16 part), about a 19-bit significand, so the
 //         rounding of the leapfrog-integrated fields does not random-walk.

@@ -85,7 +85,7 @@ def test_fp64_fwd64_matches_oracle_f64(order):
 def test_half_precision_fwd64_vs_fp64_oracle(precision, bar):
     """Receiver traces of the two 16-bit storage modes against the FP64 oracle, 500 steps.
     fp16c (binary16 + 8-bit residual on v, tau, p) must meet 1e-2; plain binary16 storage
-    (the reference's `half` arrays, with float C-PML memory) is held to its measured level."""
+    (every array binary16, as the reference's `half` arrays) is held to its measured level."""
     print(f"fwd64 {precision} order 4", flush=True)
     cfg = configs.fwd64(order=4)
     nt = cfg["nt"]
