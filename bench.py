@@ -34,10 +34,13 @@ def slab_counts(n, L, R, h):
     return lo + hi
 
 
-# element bytes per storage mode: field, coefficient, C-PML carry, residual (FP16C only,
-# on the 10 compensated fields v, tau, p); FP16 keeps every array in binary16, FP16C keeps
-# float carries
-WIDTHS = {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 2, 0), "fp16c": (2, 2, 4, 1), "fp64": (8, 8, 8, 0)}
+# element bytes per storage mode: field, coefficient, C-PML carry, residual (FP16C on the 10
+# compensated fields v, tau, p; FP16T on the 7 of tau, p); FP16 keeps every array in
+# binary16, FP16C and FP16T keep float carries
+WIDTHS = {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 2, 0), "fp16c": (2, 2, 4, 1), "fp16t": (2, 2, 4, 1),
+          "fp64": (8, 8, 8, 0)}
+# residual accesses per updated cell (velocity kernel, stress kernel, one fused step)
+RESIDUAL_ACCESSES = {"fp16c": (13, 17, 20), "fp16t": (7, 14, 14)}
 
 
 def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):
@@ -48,17 +51,19 @@ def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):
     carries.  Stress kernel: reads 6 + 7, writes 7, reads 8 coefficients, 4 carries per
     slab.  The fused-step figure is 26 field + 22 coefficient accesses per cell plus
     16 carry accesses per slab membership (one read and one write of 8 carries); FP16C adds
-    one residual byte per access of v, tau, p (V: 10 reads + 3 writes, T: 10 + 7, step 20)."""
+    one residual byte per access of v, tau, p (V: 10 reads + 3 writes, T: 10 + 7, step 20),
+    FP16T per access of tau, p (V: 7 reads, T: 7 + 7, step 14)."""
     wf, wk, wc, wr = WIDTHS[mode]
+    rv, rt, rstep = RESIDUAL_ACCESSES.get(mode, (0, 0, 0))
     h = order // 2
     ni, nj, nk = nx - 2 * h, ny - 2 * h, nz - 2 * h
     upd = ni * nj * nk
     Lx, Rx, Ly, Ry, Lz, Rz = npml_cells
     memb = (slab_counts(nx, Lx, Rx, h) * nj * nk + slab_counts(ny, Ly, Ry, h) * ni * nk +
             slab_counts(nz, Lz, Rz, h) * ni * nj)
-    v = upd * (19 * wf + 14 * wk + 13 * wr) + memb * 8 * wc
-    t = upd * (20 * wf + 8 * wk + 17 * wr) + memb * 8 * wc
-    step = upd * (26 * wf + 22 * wk + 20 * wr) + memb * 16 * wc
+    v = upd * (19 * wf + 14 * wk + rv * wr) + memb * 8 * wc
+    t = upd * (20 * wf + 8 * wk + rt * wr) + memb * 8 * wc
+    step = upd * (26 * wf + 22 * wk + rstep * wr) + memb * 16 * wc
     return v, t, step, upd, memb
 
 
@@ -350,7 +355,7 @@ def main():
                           "bytes_per_cell_update": round(bstep / upd, 2)},
         "finite": r32["finite"],
     }
-    for mode in ([] if args.no_fp16 else ["fp16c", "fp16"]):
+    for mode in ([] if args.no_fp16 else ["fp16c", "fp16t", "fp16"]):
         r16 = run_precision(cfg, mode, args.steps, args.warmup)
         v16 = n * args.steps / r16["wall_s"] / 1e6
         _, _, bstep16, _, _ = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"],

@@ -47,8 +47,10 @@ enum biot_precision {
   BIOT_FP16 = 1,  /* IEEE binary16 wavefields, coefficients and C-PML memory with per-field
                      scaling, float arithmetic */
   BIOT_FP64 = 2,  /* double storage and arithmetic (accuracy reference on the GPU) */
-  BIOT_FP16C = 3  /* as BIOT_FP16, plus an 8-bit residual per value of v, tau and p
-                     (value = h + r * ulp(h)/256): compensated half precision */
+  BIOT_FP16C = 3, /* as BIOT_FP16 with float C-PML memory, plus an 8-bit residual per value of
+                     v, tau and p (value = h + r * ulp(h)/256): compensated half precision */
+  BIOT_FP16T = 4  /* as BIOT_FP16C with the residual on tau and p only (velocities plain
+                     binary16): faster, with a larger error than BIOT_FP16C */
 };
 
 enum biot_field {

@@ -52,13 +52,13 @@ double half_res_unit(_Float16 h)
   return std::ldexp(1.0, e - 33);
 }
 
-// host storage-type conversion; lo: FP16C residuals of a compensated field, else null
+// host storage-type conversion; lo: the residuals of a compensated field, else null
 void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst, int8_t *lo)
 {
   if (prec == BIOT_FP32) {
     float *d = (float *)dst;
     for (int64_t c = 0; c < n; ++c) d[c] = (float)(src[c] * scale_inv);
-  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C) {
+  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T) {
     _Float16 *d = (_Float16 *)dst;
     for (int64_t c = 0; c < n; ++c) {
       const float x = (float)(src[c] * scale_inv);
@@ -78,7 +78,7 @@ void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double
   if (prec == BIOT_FP32) {
     const float *s = (const float *)src;
     for (int64_t c = 0; c < n; ++c) dst[c] = (double)s[c] * scale;
-  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C) {
+  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T) {
     const _Float16 *s = (const _Float16 *)src;
     for (int64_t c = 0; c < n; ++c) {
       double v = (double)(float)s[c];
@@ -136,8 +136,8 @@ static int validate_grid(const biot_grid *g, std::string &why)
     why = "grid too large";
     return BIOT_ERR_ARG;
   }
-  if (g->precision < 0 || g->precision > 3) {
-    why = "precision must be BIOT_FP32, BIOT_FP16, BIOT_FP64 or BIOT_FP16C";
+  if (g->precision < 0 || g->precision > 4) {
+    why = "precision must be BIOT_FP32, BIOT_FP16, BIOT_FP64, BIOT_FP16C or BIOT_FP16T";
     return BIOT_ERR_ARG;
   }
   if (!(g->dx > 0) || !(g->dt > 0) || !(g->Cv > 0) || !(g->Cp > 0)) {

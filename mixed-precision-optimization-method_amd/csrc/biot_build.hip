@@ -241,7 +241,8 @@ hipError_t launch_ref_to_kernel(int prec, const RefArgs &a, hipStream_t s)
     case PREC_FP32: hipLaunchKernelGGL(k_ref_to_kernel<float>, grid, dim3(256), 0, s, a); break;
     case PREC_FP64: hipLaunchKernelGGL(k_ref_to_kernel<double>, grid, dim3(256), 0, s, a); break;
     case PREC_FP16:
-    case PREC_FP16C: hipLaunchKernelGGL(k_ref_to_kernel<_Float16>, grid, dim3(256), 0, s, a); break;
+    case PREC_FP16C:
+    case PREC_FP16T: hipLaunchKernelGGL(k_ref_to_kernel<_Float16>, grid, dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

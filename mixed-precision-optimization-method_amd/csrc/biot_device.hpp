@@ -14,7 +14,8 @@
 //                 psi_n = c + a D_n,   D <- D + psi_n,   c <- e (psi_n + a D_n)
 //             which is algebraically the recursion of kernel.cu:188-196.
 //   profiles: per axis e and a = -dt d / 2 at integer and half nodes (1-D, arithmetic type).
-//   residuals (FP16C only): one int8 array per compensated field (v, tau, p).
+//   residuals (FP16C, FP16T): one int8 array per compensated field (FP16C v, tau, p;
+//             FP16T tau, p).
 //
 // Storage modes (enum biot_precision):
 //   FP32  fields, coefficients, carries float; arithmetic float.
@@ -22,6 +23,9 @@
 //   FP16C as FP16, plus an int8 residual r per value of v, tau and p: the value is
 //         h + r * ulp(h) / 256 (h the binary16 part), about a 19-bit significand, so the
 //         rounding of the leapfrog-integrated fields does not random-walk.
+//   FP16T as FP16C with the residual on tau and p only (the velocities plain binary16): the
+//         stresses' rounding dominates the binary16 error, and the velocity residuals are
+//         a third of FP16C's residual traffic.
 //   FP64  everything double.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -31,9 +35,13 @@
 namespace biot {
 
 enum { NFIELD = 13, NKC = 22, NCARRY = 8 };
-enum { PREC_FP32 = 0, PREC_FP16 = 1, PREC_FP64 = 2, PREC_FP16C = 3 };
-// fields carrying an FP16C residual: vx vy vz txx tyy tzz txy txz tyz p
-constexpr uint32_t COMP_FIELDS = 0x1FC7u;
+enum { PREC_FP32 = 0, PREC_FP16 = 1, PREC_FP64 = 2, PREC_FP16C = 3, PREC_FP16T = 4 };
+// fields carrying a residual: FP16C vx vy vz txx tyy tzz txy txz tyz p; FP16T txx .. tyz p
+constexpr uint32_t COMP_FIELDS_C = 0x1FC7u, COMP_FIELDS_T = 0x1FC0u;
+constexpr uint32_t comp_fields(int prec)
+{
+  return prec == PREC_FP16C ? COMP_FIELDS_C : prec == PREC_FP16T ? COMP_FIELDS_T : 0u;
+}
 // kernel coefficient order
 enum {
   K_U1X = 0, K_U1Y, K_U1Z, K_U2X, K_U2Y, K_U2Z, K_W1X, K_W1Y, K_W1Z, K_W2X, K_W2Y, K_W2Z,
@@ -44,7 +52,7 @@ enum {
 // Kernel arguments; pointers are type-erased, their element types follow the storage mode.
 struct StepArgs {
   void *f[NFIELD];           // field storage
-  int8_t *lo[NFIELD];        // FP16C residuals (null for other modes / fields)
+  int8_t *lo[NFIELD];        // FP16C / FP16T residuals (null for other modes / fields)
   const void *k[NKC];        // coefficient arrays (field storage type)
   // the same coefficients packed per cell (storage type): kv holds K[0..13] of the velocity
   // half step, kt K[14..21] of the stress half step, cell c at element c*14 / c*8 (addressed
@@ -66,7 +74,7 @@ struct StepArgs {
 
 inline size_t field_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP32 ? 4 : 2; }
 // C-PML carries: binary16 in the FP16 policy (all state 16-bit, as the reference stores every
-// array in half), float in FP32 and FP16C, double in FP64
+// array in half), float in FP32, FP16C and FP16T, double in FP64
 inline size_t carry_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP16 ? 2 : 4; }
 
 // Host-side launchers (biot_kernels.hip).
@@ -153,8 +161,8 @@ hipError_t launch_record_batch(int prec, const RecordArgs *tab, int nshots, int6
 // *it += 1 / *it = v on the device (kernels, so they may sit in a captured graph)
 hipError_t launch_advance(int *it, hipStream_t s);
 hipError_t launch_set_counter(int *it, int v, hipStream_t s);
-// multiply a stored array by f (lo: its FP16C residuals or null); carries use prec of the
-// carry type, i.e. pass PREC_FP32 for FP16/FP16C carries
+// multiply a stored array by f (lo: its residuals or null); carries use prec of the carry
+// type, i.e. pass PREC_FP32 for FP16C/FP16T carries and PREC_FP16 for FP16 carries
 hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
 // element e of every 4-element record times f[e] (carry groups whose carries scale differently)
 hipError_t launch_rescale4(int prec, void *p, int64_t nrec, const double f[4], hipStream_t s);

@@ -206,7 +206,7 @@ void *biot_handle::field_ptr(int f) const { return (char *)d_fields + (size_t)f 
 
 int8_t *biot_handle::lo_ptr(int f) const
 {
-  if (!d_lo || !((COMP_FIELDS >> f) & 1u)) return nullptr;
+  if (!d_lo || !((comp_fields(prec) >> f) & 1u)) return nullptr;
   return d_lo + (size_t)f * n;
 }
 
@@ -257,7 +257,7 @@ int biot_handle::allocate()
   };
   bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, (size_t)NKC * n * esz) &&
             alloc(&d_kpack, (size_t)NKC * n * esz);
-  if (ok && prec == BIOT_FP16C) ok = alloc((void **)&d_lo, (size_t)NFIELD * n);
+  if (ok && comp_fields(prec)) ok = alloc((void **)&d_lo, (size_t)NFIELD * n);
   for (int ax = 0; ax < 3 && ok; ++ax) ok = alloc(&d_carry[ax], (size_t)NCARRY * slab[ax] * csz);
   const int dims[3] = {g.nx, g.ny, g.nz};
   const size_t asz = prec == BIOT_FP64 ? 8 : 4;

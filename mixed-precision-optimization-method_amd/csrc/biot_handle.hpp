@@ -22,7 +22,7 @@ struct biot_handle {
 
   // device memory
   void *d_fields = nullptr;          // 13 * n
-  int8_t *d_lo = nullptr;            // FP16C residuals, 13 * n (compensated fields used)
+  int8_t *d_lo = nullptr;            // FP16C / FP16T residuals, 13 * n (compensated fields used)
   void *d_coef = nullptr;            // 22 * n
   void *d_kpack = nullptr;           // the coefficients packed per cell (velocity, then stress block)
   void *d_carry[3] = {nullptr, nullptr, nullptr};
@@ -61,7 +61,7 @@ struct biot_handle {
   ~biot_handle();
   void *field_ptr(int f) const;
   int8_t *lo_ptr(int f) const;  // residual array of field f, or null
-  bool half() const { return prec == BIOT_FP16 || prec == BIOT_FP16C; }
+  bool half() const { return prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T; }
   void fill_args(biot::StepArgs &a) const;
   int allocate();
   int upload_profiles();

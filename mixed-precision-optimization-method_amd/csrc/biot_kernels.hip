@@ -26,29 +26,41 @@ namespace {
 
 using u32 = uint32_t;
 
+// cmask: the fields that carry a residual
 struct P32 {
   using S = float;
   using SC = float;
   using A = float;
   static constexpr bool comp = false, scaled = false;
+  static constexpr uint32_t cmask = 0u;
 };
 struct P16 {
   using S = _Float16;
   using SC = _Float16;
   using A = float;
   static constexpr bool comp = false, scaled = true;
+  static constexpr uint32_t cmask = 0u;
 };
 struct P16C {
   using S = _Float16;
   using SC = float;
   using A = float;
   static constexpr bool comp = true, scaled = true;
+  static constexpr uint32_t cmask = COMP_FIELDS_C;
+};
+struct P16T {
+  using S = _Float16;
+  using SC = float;
+  using A = float;
+  static constexpr bool comp = true, scaled = true;
+  static constexpr uint32_t cmask = COMP_FIELDS_T;
 };
 struct P64 {
   using S = double;
   using SC = double;
   using A = double;
   static constexpr bool comp = false, scaled = false;
+  static constexpr uint32_t cmask = 0u;
 };
 
 // element idx of an array, byte offset formed in 32 bits
@@ -73,7 +85,7 @@ __device__ __forceinline__ int h_exp(_Float16 h)
 __device__ __forceinline__ float res_unit(int e) { return __builtin_bit_cast(float, (u32)(e + 94) << 23); }
 __device__ __forceinline__ float res_inv(int e) { return __builtin_bit_cast(float, (u32)(160 - e) << 23); }
 
-// value = stored part (+ residual for compensated fields of the FP16C policy)
+// value = stored part (+ residual for the compensated fields of the FP16C / FP16T policies)
 template <class P>
 __device__ __forceinline__ typename P::A load_raw(const void *f, const int8_t *lo, bool comp_f,
                                                   u32 idx)
@@ -103,12 +115,12 @@ __device__ __forceinline__ void store_raw(void *f, int8_t *lo, bool comp_f, u32 
 template <class P, class AT>
 __device__ __forceinline__ typename P::A load_rt(const AT &a, int f, u32 idx)
 {
-  return load_raw<P>(a.f[f], a.lo[f], (COMP_FIELDS >> f) & 1u, idx);
+  return load_raw<P>(a.f[f], a.lo[f], (P::cmask >> f) & 1u, idx);
 }
 template <class P, class AT>
 __device__ __forceinline__ void store_rt(const AT &a, int f, u32 idx, typename P::A x)
 {
-  store_raw<P>(a.f[f], a.lo[f], (COMP_FIELDS >> f) & 1u, idx, x);
+  store_raw<P>(a.f[f], a.lo[f], (P::cmask >> f) & 1u, idx, x);
 }
 // compile-time field index: the residual test folds away
 template <class P, int F, class AT>
@@ -595,7 +607,7 @@ __device__ __forceinline__ void source_cell(const SourceArgs &sa, int it)
   const int f = threadIdx.x;
   if (f >= NFIELD || !(sa.mask & (1u << f))) return;
   const u32 c = (u32)sa.cell;
-  const bool cf = (COMP_FIELDS >> f) & 1u;
+  const bool cf = (P::cmask >> f) & 1u;
   const A amp = (A)sa.wavelet[it] * (A)sa.inv_scale[f];
   store_raw<P>(sa.f[f], sa.lo[f], cf, c, load_raw<P>(sa.f[f], sa.lo[f], cf, c) + amp);
 }
@@ -612,7 +624,7 @@ __device__ __forceinline__ void record_one(const RecordArgs &ra, int64_t r, int 
   if (r >= ra.nrec || ci >= ra.ncomp) return;
   const int f = ra.comp[ci];
   // the product of two floats is exact in double, so (float)v is the rounded float product
-  const double v = (double)load_raw<P>(ra.f[f], ra.lo[f], (COMP_FIELDS >> f) & 1u, (u32)ra.rec[r]) *
+  const double v = (double)load_raw<P>(ra.f[f], ra.lo[f], (P::cmask >> f) & 1u, (u32)ra.rec[r]) *
                    (double)ra.scale[ci];
   const int64_t o = ((int64_t)ci * ra.nt_max + it) * ra.nrec + r;
   ra.traces[o] = (float)v;
@@ -749,6 +761,7 @@ hipError_t dispatch_order(int prec, const StepArgs &a, LaunchShape sh, hipStream
     case PREC_FP32: return launch_step<P32, ORDER, VEL>(a, sh, s);
     case PREC_FP16: return launch_step<P16, ORDER, VEL>(a, sh, s);
     case PREC_FP16C: return launch_step<P16C, ORDER, VEL>(a, sh, s);
+    case PREC_FP16T: return launch_step<P16T, ORDER, VEL>(a, sh, s);
     case PREC_FP64: return launch_step<P64, ORDER, VEL>(a, sh, s);
   }
   return hipErrorInvalidValue;
@@ -851,6 +864,7 @@ hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s)
     case PREC_FP32: hipLaunchKernelGGL(k_source<P32>, dim3(1), dim3(64), 0, s, a); break;
     case PREC_FP16: hipLaunchKernelGGL(k_source<P16>, dim3(1), dim3(64), 0, s, a); break;
     case PREC_FP16C: hipLaunchKernelGGL(k_source<P16C>, dim3(1), dim3(64), 0, s, a); break;
+    case PREC_FP16T: hipLaunchKernelGGL(k_source<P16T>, dim3(1), dim3(64), 0, s, a); break;
     case PREC_FP64: hipLaunchKernelGGL(k_source<P64>, dim3(1), dim3(64), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
@@ -867,6 +881,7 @@ hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hi
       break;
     case PREC_FP16:
     case PREC_FP16C:
+    case PREC_FP16T:
       hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, lo, n, f);
       break;
     case PREC_FP64:
@@ -886,6 +901,7 @@ hipError_t launch_slice_xz(int prec, const void *f, const int8_t *lo, bool comp,
     case PREC_FP32: hipLaunchKernelGGL(k_slice_xz<P32>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     case PREC_FP16: hipLaunchKernelGGL(k_slice_xz<P16>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     case PREC_FP16C: hipLaunchKernelGGL(k_slice_xz<P16C>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
+    case PREC_FP16T: hipLaunchKernelGGL(k_slice_xz<P16T>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     case PREC_FP64: hipLaunchKernelGGL(k_slice_xz<P64>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     default: return hipErrorInvalidValue;
   }
@@ -903,6 +919,7 @@ hipError_t launch_pack_coef(int prec, const void *soa, void *kv, void *kt, int64
       break;
     case PREC_FP16:
     case PREC_FP16C:
+    case PREC_FP16T:
       hipLaunchKernelGGL(k_pack_coef<_Float16>, dim3(blocks), dim3(256), 0, s, (const _Float16 *)soa,
                          (_Float16 *)kv, (_Float16 *)kt, n);
       break;
@@ -922,6 +939,7 @@ hipError_t launch_rescale4(int prec, void *p, int64_t nrec, const double f[4], h
   switch (prec) {
     case PREC_FP32:
     case PREC_FP16C:
+    case PREC_FP16T:
       hipLaunchKernelGGL(k_rescale4<float>, dim3(blocks), dim3(256), 0, s, (float *)p, nrec, f[0], f[1], f[2], f[3]);
       break;
     case PREC_FP16:
@@ -944,6 +962,7 @@ hipError_t launch_record(int prec, const RecordArgs &a, hipStream_t s)
     case PREC_FP32: hipLaunchKernelGGL(k_record<P32>, grid, block, 0, s, a); break;
     case PREC_FP16: hipLaunchKernelGGL(k_record<P16>, grid, block, 0, s, a); break;
     case PREC_FP16C: hipLaunchKernelGGL(k_record<P16C>, grid, block, 0, s, a); break;
+    case PREC_FP16T: hipLaunchKernelGGL(k_record<P16T>, grid, block, 0, s, a); break;
     case PREC_FP64: hipLaunchKernelGGL(k_record<P64>, grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }

@@ -138,7 +138,7 @@ __device__ __forceinline__ void ldv(const StepArgs &a, u32 idx, typename P::A (&
       v[e] = (float)hs[e];
     }
   }
-  if constexpr (P::comp && ((COMP_FIELDS >> F) & 1u)) {
+  if constexpr (P::comp && ((COMP_FIELDS_C >> F) & 1u)) {
     constexpr int NB = ((N + RO + 3) / 4) * 4;  // residual bytes loaded
     uint32_t r[NB / 4];
     ld_dw<NB / 4>(a.lo[F], idx - (u32)RO, r);
@@ -171,7 +171,7 @@ __device__ __forceinline__ void stv(const StepArgs &a, u32 idx, const typename P
     _Float16 hs[XV];
 #pragma unroll
     for (int e = 0; e < XV; ++e) hs[e] = (_Float16)x[e];
-    constexpr bool cf = P::comp && ((COMP_FIELDS >> F) & 1u);
+    constexpr bool cf = P::comp && ((COMP_FIELDS_C >> F) & 1u);
     int8_t rs[XV];
     if constexpr (cf) {
 #pragma unroll

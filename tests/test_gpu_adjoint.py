@@ -200,9 +200,11 @@ def test_adjoint_argument_errors():
         s.adjoint(np.zeros((s.ncomp, s.nt_max, s.nrec), np.float32))
     with pytest.raises(BiotError):  # more steps than the receivers hold
         s.forward_checkpointed(cfg["nt"] + 1, 10)
-    s16 = configs.make_solver(cfg, precision="fp16c")
-    with pytest.raises(BiotError):  # the adjoint path reads FP32, FP64 or plain FP16 states
-        s16.forward_checkpointed(cfg["nt"], 10)
+    for prec in ("fp16c", "fp16t"):
+        s16 = configs.make_solver(cfg, precision=prec)
+        with pytest.raises(BiotError):  # the adjoint path reads FP32, FP64 or plain FP16 states
+            s16.forward_checkpointed(cfg["nt"], 10)
+        s16.close()
     s.forward_checkpointed(cfg["nt"], 10)
     with pytest.raises(BiotError):  # parameter gradients need biot_set_params
         s.adjoint(np.zeros((s.ncomp, s.nt_max, s.nrec), np.float32), params=True)

@@ -27,7 +27,8 @@ def test_streamed_traces_equal_plain_run(chunk):
     assert np.array_equal(b[:, 100:], ref[:, 100:])
 
 
-@pytest.mark.parametrize("precision,chunk", [("fp32", 7), ("fp32", 64), ("fp16", 50), ("fp16c", 500)])
+@pytest.mark.parametrize("precision,chunk",
+                         [("fp32", 7), ("fp32", 64), ("fp16", 50), ("fp16c", 500), ("fp16t", 64)])
 def test_streamed_snapshots_equal_plain_run(precision, chunk):
     print(f"stream snapshots {precision} chunk {chunk}", flush=True)
     cfg = configs.fwd64(order=4, nt=150)

@@ -81,11 +81,12 @@ def test_fp64_fwd64_matches_oracle_f64(order):
     assert max(errs.values()) <= 1e-10, errs
 
 
-@pytest.mark.parametrize("precision,bar", [("fp16c", 1e-2), ("fp16", 2.5e-2)])
+@pytest.mark.parametrize("precision,bar", [("fp16c", 1e-2), ("fp16t", 1e-2), ("fp16", 2.5e-2)])
 def test_half_precision_fwd64_vs_fp64_oracle(precision, bar):
-    """Receiver traces of the two 16-bit storage modes against the FP64 oracle, 500 steps.
-    fp16c (binary16 + 8-bit residual on v, tau, p) must meet 1e-2; plain binary16 storage
-    (every array binary16, as the reference's `half` arrays) is held to its measured level."""
+    """Receiver traces of the 16-bit storage modes against the FP64 oracle, 500 steps.
+    fp16c (binary16 + 8-bit residual on v, tau, p) and fp16t (residual on tau, p) must meet
+    1e-2; plain binary16 storage (every array binary16, as the reference's `half` arrays) is
+    held to its measured level."""
     print(f"fwd64 {precision} order 4", flush=True)
     cfg = configs.fwd64(order=4)
     nt = cfg["nt"]
@@ -98,7 +99,7 @@ def test_half_precision_fwd64_vs_fp64_oracle(precision, bar):
     assert max(errs) <= bar
 
 
-@pytest.mark.parametrize("precision,bar", [("fp16c", 1e-2), ("fp16", 5e-2)])
+@pytest.mark.parametrize("precision,bar", [("fp16c", 1e-2), ("fp16t", 1e-2), ("fp16", 5e-2)])
 def test_half_precision_perturbed_medium_vs_gpu_fp64(precision, bar):
     """Layered medium with +-10 % Gaussian-correlated perturbations of Kb, mu, phi (the
     512 x 512 x 384 workload's medium statistics) on a 192 x 192 x 160 grid, 1200 steps,
@@ -121,8 +122,8 @@ def test_half_precision_perturbed_medium_vs_gpu_fp64(precision, bar):
 
 def test_half_precision_full_size_perturbed512():
     """The 512 x 512 x 384, 3000-step perturbed-medium workload at full size: FP16C (binary16
-    + 8-bit residual on v, tau, p) within 1e-2 of the GPU FP64 path on the 128 x 128 surface
-    receivers; plain FP16 is reported."""
+    + 8-bit residual on v, tau, p) and FP16T (residual on tau, p) within 1e-2 of the GPU FP64
+    path on the 128 x 128 surface receivers; plain FP16 is reported."""
     import time
 
     print("perturbed512 full size", flush=True)
@@ -130,7 +131,7 @@ def test_half_precision_full_size_perturbed512():
     cfg = configs.perturbed512()
     print(f"  medium built in {time.time() - t0:.0f} s", flush=True)
     traces = {}
-    for prec in ("fp64", "fp16c", "fp16"):
+    for prec in ("fp64", "fp16c", "fp16t", "fp16"):
         t0 = time.time()
         s = configs.make_solver(cfg, precision=prec)
         s.run(cfg["nt"])
@@ -139,11 +140,11 @@ def test_half_precision_full_size_perturbed512():
         s.close()
         print(f"  {prec}: {ms / 1e3:.1f} s loop, {time.time() - t0:.0f} s total", flush=True)
     ref = traces["fp64"]
-    errs = {p: [rel_l2(traces[p][c], ref[c]) for c in range(3)] for p in ("fp16c", "fp16")}
+    errs = {p: [rel_l2(traces[p][c], ref[c]) for c in range(3)] for p in ("fp16c", "fp16t", "fp16")}
     for p, e in errs.items():
         print(f"  {p} traces rel L2 vs fp64 (vx, vz, p) = " + " ".join(f"{x:.3e}" for x in e))
-    assert np.isfinite(traces["fp16c"]).all() and np.isfinite(traces["fp16"]).all()
-    assert max(errs["fp16c"]) <= 1e-2
+    assert all(np.isfinite(traces[p]).all() for p in ("fp16c", "fp16t", "fp16"))
+    assert max(errs["fp16c"]) <= 1e-2 and max(errs["fp16t"]) <= 1e-2
 
 
 def _random_state(shape, seed):
@@ -182,7 +183,7 @@ def test_full_size_random_state_matches_oracle(precision, mode, tol):
     s.close()
 
 
-@pytest.mark.parametrize("precision,tol", [("fp16", 3e-2), ("fp16c", 1e-3)])
+@pytest.mark.parametrize("precision,tol", [("fp16", 3e-2), ("fp16c", 1e-3), ("fp16t", 1e-2)])
 def test_storage_scale_change_mid_run(precision, tol):
     """A new source amplitude changes the 16-bit storage scales, and the handle re-expresses
     every stored field and C-PML carry in the new units.  A source that is silent after step
