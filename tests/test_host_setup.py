@@ -90,17 +90,20 @@ def test_argument_errors_are_status_codes():
     assert L.biot_create(C.byref(bad), C.byref(h)) == 1
     bad.nx, bad.npml = 64, 40
     assert L.biot_create(C.byref(bad), C.byref(h)) == 1
-    bad.npml, bad.precision = 10, 7
-    assert L.biot_create(C.byref(bad), C.byref(h)) == 1
+    for prec in (-1, 5, 7):  # BIOT_FP32 .. BIOT_FP16T are 0..4
+        bad.npml, bad.precision = 10, prec
+        assert L.biot_create(C.byref(bad), C.byref(h)) == 1
     assert L.biot_ricker(C.c_float(-1.0), C.c_float(1e-3), 4, None) == 1
     assert L.biot_status_string(4) == b"non-finite value detected"
 
 
-def test_create_without_device_fails_loudly():
-    """Without a GPU the library reports a HIP error instead of falling back to the CPU."""
+@pytest.mark.parametrize("precision", [0, 1, 2, 3, 4])
+def test_create_without_device_fails_loudly(precision):
+    """Without a GPU the library reports a HIP error instead of falling back to the CPU (every
+    storage mode passes the argument checks and reaches the device)."""
     L = _lib.lib()
     h = C.c_void_p()
-    g = Grid(nx=16, ny=16, nz=16, order=2, precision=0, npml=2, dx=5, dt=4e-4, pml_v0=4000,
+    g = Grid(nx=16, ny=16, nz=16, order=2, precision=precision, npml=2, dx=5, dt=4e-4, pml_v0=4000,
              pml_R=1e-6, Cv=1e8, Cp=1, device=0)
     rc = L.biot_create(C.byref(g), C.byref(h))
     if rc == BIOT_OK:
