@@ -257,6 +257,21 @@ def stream_copy_rate(mib=1024):
     return json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"]
 
 
+def hip_copy_rates(mib=1024):
+    """HBM rate (GB/s, bytes read + written) of the library's own streaming copy (one 16-byte
+    load and store per thread, 20 timed launches), with plain and with nontemporal accesses."""
+    import ctypes as C
+
+    from biot_amd import _lib
+
+    out = {}
+    for name, variant in (("plain", 0), ("nontemporal", 1)):
+        g = C.c_double()
+        rc = _lib.lib().biot_copy_bandwidth(0, mib * 2 ** 20, 20, variant, C.byref(g))
+        out[name] = round(g.value, 1) if rc == 0 else None
+    return out
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB (torch's copy kernel), repeated 5 times:
     the HBM-traffic counters of this dispatch calibrate FETCH_SIZE / WRITE_SIZE."""
@@ -311,6 +326,9 @@ def main():
     from biot_amd import configs
 
     copy_gbs = stream_copy_rate()  # before this process touches the GPU
+    hip_copy = hip_copy_rates()
+    log(f"[bench] plain copies: torch {copy_gbs} GB/s, HIP 16-byte {hip_copy['plain']} GB/s, "
+        f"nontemporal {hip_copy['nontemporal']} GB/s")
     cfg = configs.layered256(order=4, nt=args.warmup + args.steps, n=args.size)
     n = cfg["nx"] * cfg["ny"] * cfg["nz"]
     r32 = run_precision(cfg, "fp32", args.steps, args.warmup)
@@ -350,6 +368,7 @@ def main():
                           "frac": round(ach_t / HBM_PEAK_GBS, 4), "algorithmic_bytes": bt,
                           "mean_launch_ms": round(r32["kernel_ms_t"], 5)},
         "stream_copy_GBs": copy_gbs,
+        "hip_copy_GBs": hip_copy,
         "step_roofline": {"achieved_GBs": round(ach_step, 1), "frac": round(ach_step / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes_per_step": bstep,
                           "bytes_per_cell_update": round(bstep / upd, 2)},

@@ -312,6 +312,12 @@ int biot_get_field_scales(biot_handle *h, float *scales13);
 /* Device bytes held by the handle. */
 int biot_device_bytes(biot_handle *h, int64_t *bytes);
 
+/* Measurement aid (no counterpart in the reference): the HBM rate, in GB/s of bytes read plus
+ * bytes written, of a plain device copy of `bytes` bytes (a multiple of 16, at most 16 GiB)
+ * with one 16-byte load and store per thread, timed over `reps` launches after 3 warm-up
+ * launches on device `device` (made current); variant 0 plain loads/stores, 1 nontemporal. */
+int biot_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double *GBs);
+
 #ifdef __cplusplus
 }
 #endif
