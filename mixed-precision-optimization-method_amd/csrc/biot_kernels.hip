@@ -26,13 +26,15 @@ namespace {
 
 using u32 = uint32_t;
 
-// cmask: the fields that carry a residual
+// cmask: the fields that carry a residual; nt: the fields' own-value loads and stores are
+// nontemporal (measured +2 % for FP32, -3 % for the residual policies, neutral for FP16)
 struct P32 {
   using S = float;
   using SC = float;
   using A = float;
   static constexpr bool comp = false, scaled = false;
   static constexpr uint32_t cmask = 0u;
+  static constexpr bool nt = true;
 };
 struct P16 {
   using S = _Float16;
@@ -40,6 +42,7 @@ struct P16 {
   using A = float;
   static constexpr bool comp = false, scaled = true;
   static constexpr uint32_t cmask = 0u;
+  static constexpr bool nt = false;
 };
 struct P16C {
   using S = _Float16;
@@ -47,6 +50,7 @@ struct P16C {
   using A = float;
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_C;
+  static constexpr bool nt = false;
 };
 struct P16T {
   using S = _Float16;
@@ -54,6 +58,7 @@ struct P16T {
   using A = float;
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_T;
+  static constexpr bool nt = false;
 };
 struct P64 {
   using S = double;
@@ -61,6 +66,7 @@ struct P64 {
   using A = double;
   static constexpr bool comp = false, scaled = false;
   static constexpr uint32_t cmask = 0u;
+  static constexpr bool nt = false;
 };
 
 // element idx of an array, byte offset formed in 32 bits
@@ -98,13 +104,28 @@ __device__ __forceinline__ typename P::A load_raw(const void *f, const int8_t *l
   }
   return v;
 }
+// the value of a cell that only the updating thread reads (its own old value)
+template <class P>
+__device__ __forceinline__ typename P::A load_own(const void *f, const int8_t *lo, bool comp_f, u32 idx)
+{
+  using S = typename P::S;
+  S h;
+  if constexpr (P::nt) h = __builtin_nontemporal_load(&at<S>(f, idx));
+  else h = at<S>(f, idx);
+  typename P::A v = (typename P::A)h;
+  if constexpr (P::comp) {
+    if (comp_f) v += (float)at<int8_t>(lo, idx) * res_unit(h_exp(h));
+  }
+  return v;
+}
 template <class P>
 __device__ __forceinline__ void store_raw(void *f, int8_t *lo, bool comp_f, u32 idx,
                                           typename P::A x)
 {
   using S = typename P::S;
   const S h = (S)x;
-  at<S>(f, idx) = h;
+  if constexpr (P::nt) __builtin_nontemporal_store(h, &at<S>(f, idx));
+  else at<S>(f, idx) = h;
   if constexpr (P::comp) {
     if (comp_f) {
       const float r = rintf((x - (float)h) * res_inv(h_exp(h)));
@@ -127,6 +148,11 @@ template <class P, int F, class AT>
 __device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
 {
   return load_rt<P>(a, F, idx);
+}
+template <class P, int F, class AT>
+__device__ __forceinline__ typename P::A LO(const AT &a, u32 idx)
+{
+  return load_own<P>(a.f[F], a.lo[F], (P::cmask >> F) & 1u, idx);
 }
 template <class P, int F, class AT>
 __device__ __forceinline__ void ST(const AT &a, u32 idx, typename P::A x)
@@ -200,7 +226,8 @@ __device__ __forceinline__ typename P::A pml(typename P::SC &cv, typename P::A D
   return D + psi;
 }
 // the record of four carries of slab cell o in group g (0: velocity half step, 1: stress half
-// step): one vector load and one vector store
+// step): one vector load and one vector store (ordinary: nontemporal accesses measured slower
+// here, the records are reused from the memory-side cache between steps)
 template <class SC>
 __device__ __forceinline__ void carry_load(SC (&cv)[4], const void *base, u32 g, u32 o, u32 sl)
 {
@@ -326,8 +353,8 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     A z3 = dbw<P, FTYZ, ORDER>(a, c, sy, tyz_c);
     A s3 = dwin<ORDER>(wp);
 
-    const A qx0 = L<P, FQX>(a, c), qy0 = L<P, FQY>(a, c), qz0 = L<P, FQZ>(a, c);
-    const A vx0 = L<P, FVX>(a, c), vy0 = L<P, FVY>(a, c), vz0 = L<P, FVZ>(a, c);
+    const A qx0 = LO<P, FQX>(a, c), qy0 = LO<P, FQY>(a, c), qz0 = LO<P, FQZ>(a, c);
+    const A vx0 = LO<P, FVX>(a, c), vy0 = LO<P, FVY>(a, c), vz0 = LO<P, FVZ>(a, c);
     A kv[14];  // K[0..13]: U1 xyz, U2 xyz, W1 xyz, W2 xyz, W3x, W3y
     coef_block<P, 14>(a, a.kv, c, 0, kv);
     const A w1x = kv[K_W1X], w2x = kv[K_W2X], w3x = kv[K_W3X];
@@ -478,9 +505,9 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     A uyz = dwin<ORDER>(wvy);
     A uzy = dfw<P, FVZ, ORDER>(a, c, sy, vz_c);
 
-    const A p0 = L<P, FP>(a, c), txx0 = L<P, FTXX>(a, c), tyy0 = L<P, FTYY>(a, c);
-    const A tzz0 = L<P, FTZZ>(a, c), txy0 = L<P, FTXY>(a, c), tyz0 = L<P, FTYZ>(a, c);
-    const A txz0 = L<P, FTXZ>(a, c);
+    const A p0 = LO<P, FP>(a, c), txx0 = LO<P, FTXX>(a, c), tyy0 = LO<P, FTYY>(a, c);
+    const A tzz0 = LO<P, FTZZ>(a, c), txy0 = LO<P, FTXY>(a, c), tyz0 = LO<P, FTYZ>(a, c);
+    const A txz0 = LO<P, FTXZ>(a, c);
     A kt[8];  // K[14..21]: P1, P2, S1, S2, S3, Sxy, Sxz, Syz
     coef_block<P, 8>(a, a.kt, c, K_P1, kt);
     const A s1 = kt[K_S1 - K_P1], s2 = kt[K_S2 - K_P1], s3 = kt[K_S3 - K_P1];
