@@ -97,6 +97,20 @@ def test_argument_errors_are_status_codes():
     assert L.biot_status_string(4) == b"non-finite value detected"
 
 
+def test_copy_bandwidth_argument_errors_without_device():
+    """The streaming-copy measurement rejects bad sizes, counts and variants before it touches
+    a device (status 1); with valid arguments and no GPU it reports a HIP error (2)."""
+    L, g = _lib.lib(), C.c_double()
+    for args in ((0, 100, 5, 0), (0, 0, 5, 0), (0, 1 << 20, 0, 0), (0, 1 << 20, 1001, 0), (0, 1 << 20, 5, 2),
+                 (0, (16 << 30) + 16, 5, 0)):
+        assert L.biot_copy_bandwidth(*args, C.byref(g)) == 1
+    assert L.biot_copy_bandwidth(0, 1 << 20, 5, 0, None) == 1
+    rc = L.biot_copy_bandwidth(0, 1 << 20, 1, 0, C.byref(g))
+    if rc == BIOT_OK:
+        pytest.skip("a GPU is present")
+    assert rc == 2
+
+
 @pytest.mark.parametrize("precision", [0, 1, 2, 3, 4])
 def test_create_without_device_fails_loudly(precision):
     """Without a GPU the library reports a HIP error instead of falling back to the CPU (every
