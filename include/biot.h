@@ -313,9 +313,13 @@ int biot_get_field_scales(biot_handle *h, float *scales13);
 int biot_device_bytes(biot_handle *h, int64_t *bytes);
 
 /* Measurement aid (no counterpart in the reference): the HBM rate, in GB/s of bytes read plus
- * bytes written, of a plain device copy of `bytes` bytes (a multiple of 16, at most 16 GiB)
- * with one 16-byte load and store per thread, timed over `reps` launches after 3 warm-up
- * launches on device `device` (made current); variant 0 plain loads/stores, 1 nontemporal. */
+ * bytes written, of a streaming kernel timed over `reps` launches after 3 warm-up launches on
+ * device `device` (made current).  Variants 0 and 1: a device copy of `bytes` bytes (a
+ * multiple of 16, at most 16 GiB), one 16-byte load and store per thread, ordinary (0) or
+ * nontemporal (1) accesses.  Variants 2 and 3: the FP32 velocity kernel's access pattern
+ * without its stencil, over bytes/4 cells (bytes at most 1 GiB): 13 float fields and a
+ * 56-byte packed block read, 6 float fields written per cell (132 B), ordinary (2) or
+ * nontemporal (3) stores. */
 int biot_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double *GBs);
 
 #ifdef __cplusplus

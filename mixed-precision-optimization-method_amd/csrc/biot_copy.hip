@@ -1,6 +1,8 @@
-// A plain streaming copy on the device, timed with HIP events: the HBM rate (bytes read +
-// bytes written per second) that a kernel doing nothing but 16-byte coalesced loads and
-// stores reaches on this GPU.  The stencil kernels' achieved rates are read against it.
+// Streaming kernels on the device, timed with HIP events: the HBM rate (bytes read + bytes
+// written per second) of a plain 16-byte copy, and of the FP32 velocity kernel's access
+// pattern without its stencil (13 float fields read, the 56-byte packed coefficient block of
+// 3 x 16 + 8 bytes read, 6 float fields written per cell, all at the cell's own index).  The
+// stencil kernels' achieved rates are read against these.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -21,28 +23,76 @@ __global__ void __launch_bounds__(256) k_copy16(const dw4 *__restrict__ src, dw4
   else dst[i] = src[i];
 }
 
+struct VPattern {
+  const float *r[13];
+  float *w[6];
+  const dw4 *k4[3];
+  const uint64_t *k2;
+  int64_t n;
+};
+// the velocity kernel's streams at one cell per thread; the loaded values feed the stores
+template <bool NT>
+__global__ void __launch_bounds__(256) k_vpattern(const VPattern a)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 13; ++k) s += a.r[k][i];
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    const dw4 v = __builtin_nontemporal_load(a.k4[g] + i);
+    s += __builtin_bit_cast(float, v.x ^ v.y ^ v.z ^ v.w);
+  }
+  const uint64_t t = __builtin_nontemporal_load(a.k2 + i);
+  s += __builtin_bit_cast(float, (uint32_t)t ^ (uint32_t)(t >> 32));
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if constexpr (NT) __builtin_nontemporal_store(s + (float)k, a.w[k] + i);
+    else a.w[k][i] = s + (float)k;
+  }
+}
+
 }  // namespace
 
 extern "C" int biot_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double *GBs)
 {
   if (!GBs || bytes < 16 || bytes % 16 || bytes > ((int64_t)16 << 30) || reps < 1 || reps > 1000 ||
-      (variant != 0 && variant != 1))
+      variant < 0 || variant > 3)
     return BIOT_ERR_ARG;
+  if (variant >= 2 && bytes > ((int64_t)1 << 30)) return BIOT_ERR_ARG;  // 33 x bytes allocated
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return BIOT_ERR_HIP;
   if (hipSetDevice(device) != hipSuccess) return BIOT_ERR_HIP;
-  dw4 *src = nullptr, *dst = nullptr;
+  // copies: source and destination of `bytes`; the velocity pattern: 19 float fields of
+  // `bytes` each and a packed block of 14 floats per cell (one allocation)
+  const bool pattern = variant >= 2;
+  const int64_t n = pattern ? bytes / 4 : bytes / 16;
+  const size_t total = pattern ? (size_t)n * (19 * 4 + 56) : (size_t)bytes * 2;
+  char *mem = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = BIOT_OK;
-  if (hipMalloc((void **)&src, (size_t)bytes) != hipSuccess || hipMalloc((void **)&dst, (size_t)bytes) != hipSuccess)
-    rc = BIOT_ERR_NOMEM;
+  if (hipMalloc((void **)&mem, total) != hipSuccess) rc = BIOT_ERR_NOMEM;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = BIOT_ERR_HIP;
-  if (!rc && hipMemset(src, 1, (size_t)bytes) != hipSuccess) rc = BIOT_ERR_HIP;
-  const int64_t n16 = bytes / 16;
-  const dim3 grid((unsigned)((n16 + 255) / 256)), block(256);
+  if (!rc && hipMemset(mem, 1, total) != hipSuccess) rc = BIOT_ERR_HIP;
+  const dw4 *src = (const dw4 *)mem;
+  dw4 *dst = (dw4 *)(mem + (pattern ? 0 : bytes));
+  VPattern vp{};
+  if (pattern) {
+    for (int k = 0; k < 13; ++k) vp.r[k] = (const float *)(mem + (size_t)k * n * 4);
+    for (int k = 0; k < 6; ++k) vp.w[k] = (float *)(mem + (size_t)(13 + k) * n * 4);
+    for (int g = 0; g < 3; ++g) vp.k4[g] = (const dw4 *)(mem + (size_t)19 * n * 4 + (size_t)g * n * 16);
+    vp.k2 = (const uint64_t *)(mem + (size_t)19 * n * 4 + (size_t)3 * n * 16);
+    vp.n = n;
+  }
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
   auto launch = [&]() {
-    if (variant == 1) hipLaunchKernelGGL(k_copy16<true>, grid, block, 0, 0, src, dst, n16);
-    else hipLaunchKernelGGL(k_copy16<false>, grid, block, 0, 0, src, dst, n16);
+    switch (variant) {
+      case 0: hipLaunchKernelGGL(k_copy16<false>, grid, block, 0, 0, src, dst, n); break;
+      case 1: hipLaunchKernelGGL(k_copy16<true>, grid, block, 0, 0, src, dst, n); break;
+      case 2: hipLaunchKernelGGL(k_vpattern<false>, grid, block, 0, 0, vp); break;
+      default: hipLaunchKernelGGL(k_vpattern<true>, grid, block, 0, 0, vp); break;
+    }
   };
   if (!rc) {
     for (int r = 0; r < 3; ++r) launch();  // warm-up
@@ -55,11 +105,10 @@ extern "C" int biot_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, 
   if (!rc) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess || !(ms > 0)) rc = BIOT_ERR_HIP;
-    else *GBs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    else *GBs = (pattern ? 132.0 * (double)n : 2.0 * (double)bytes) * reps / (ms * 1e-3) / 1e9;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  if (src) (void)hipFree(src);
-  if (dst) (void)hipFree(dst);
+  if (mem) (void)hipFree(mem);
   return rc;
 }

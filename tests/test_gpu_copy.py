@@ -1,5 +1,5 @@
-"""GPU check of the streaming-copy measurement (biot_copy_bandwidth): a plausible HBM rate
-for both access variants, and argument errors as status codes."""
+"""GPU check of the streaming measurement (biot_copy_bandwidth): a plausible HBM rate for the
+copies and the velocity access pattern, and argument errors as status codes."""
 import ctypes as C
 
 import pytest
@@ -9,7 +9,7 @@ from biot_amd import _lib
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_copy_bandwidth_rate(variant):
     g = C.c_double()
     assert _lib.lib().biot_copy_bandwidth(0, 256 * 2 ** 20, 5, variant, C.byref(g)) == 0
@@ -19,6 +19,7 @@ def test_copy_bandwidth_rate(variant):
 
 def test_copy_bandwidth_arguments():
     L, g = _lib.lib(), C.c_double()
-    for args in ((0, 100, 5, 0), (0, 0, 5, 0), (0, 1 << 20, 0, 0), (0, 1 << 20, 5, 2), (-1, 1 << 20, 5, 0)):
+    for args in ((0, 100, 5, 0), (0, 0, 5, 0), (0, 1 << 20, 0, 0), (0, 1 << 20, 5, 4), (-1, 1 << 20, 5, 0),
+                 (0, (1 << 30) + 16, 5, 2)):
         assert L.biot_copy_bandwidth(*args, C.byref(g)) != 0
     assert L.biot_copy_bandwidth(0, 1 << 20, 5, 0, None) != 0
