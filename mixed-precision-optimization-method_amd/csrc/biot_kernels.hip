@@ -26,15 +26,17 @@ namespace {
 
 using u32 = uint32_t;
 
-// cmask: the fields that carry a residual; nt: the fields' own-value loads and stores are
-// nontemporal (measured +2 % for FP32, -3 % for the residual policies, neutral for FP16)
+// cmask: the fields that carry a residual; nt_v / nt_t: the velocity / stress kernel's
+// own-value loads and field stores are nontemporal (same-box A/B: FP32 velocity kernel
+// -3.5 % time, FP16 velocity kernel -1 % step time; the residual policies 2-3 % slower with
+// them, FP16's stress kernel neutral)
 struct P32 {
   using S = float;
   using SC = float;
   using A = float;
   static constexpr bool comp = false, scaled = false;
   static constexpr uint32_t cmask = 0u;
-  static constexpr bool nt = true;
+  static constexpr bool nt_v = true, nt_t = true;
 };
 struct P16 {
   using S = _Float16;
@@ -42,7 +44,7 @@ struct P16 {
   using A = float;
   static constexpr bool comp = false, scaled = true;
   static constexpr uint32_t cmask = 0u;
-  static constexpr bool nt = false;
+  static constexpr bool nt_v = true, nt_t = false;
 };
 struct P16C {
   using S = _Float16;
@@ -50,7 +52,7 @@ struct P16C {
   using A = float;
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_C;
-  static constexpr bool nt = false;
+  static constexpr bool nt_v = false, nt_t = false;
 };
 struct P16T {
   using S = _Float16;
@@ -58,7 +60,7 @@ struct P16T {
   using A = float;
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_T;
-  static constexpr bool nt = false;
+  static constexpr bool nt_v = false, nt_t = false;
 };
 struct P64 {
   using S = double;
@@ -66,7 +68,7 @@ struct P64 {
   using A = double;
   static constexpr bool comp = false, scaled = false;
   static constexpr uint32_t cmask = 0u;
-  static constexpr bool nt = false;
+  static constexpr bool nt_v = false, nt_t = false;
 };
 
 // element idx of an array, byte offset formed in 32 bits
@@ -105,12 +107,12 @@ __device__ __forceinline__ typename P::A load_raw(const void *f, const int8_t *l
   return v;
 }
 // the value of a cell that only the updating thread reads (its own old value)
-template <class P>
+template <class P, bool NT>
 __device__ __forceinline__ typename P::A load_own(const void *f, const int8_t *lo, bool comp_f, u32 idx)
 {
   using S = typename P::S;
   S h;
-  if constexpr (P::nt) h = __builtin_nontemporal_load(&at<S>(f, idx));
+  if constexpr (NT) h = __builtin_nontemporal_load(&at<S>(f, idx));
   else h = at<S>(f, idx);
   typename P::A v = (typename P::A)h;
   if constexpr (P::comp) {
@@ -118,13 +120,13 @@ __device__ __forceinline__ typename P::A load_own(const void *f, const int8_t *l
   }
   return v;
 }
-template <class P>
+template <class P, bool NT = false>
 __device__ __forceinline__ void store_raw(void *f, int8_t *lo, bool comp_f, u32 idx,
                                           typename P::A x)
 {
   using S = typename P::S;
   const S h = (S)x;
-  if constexpr (P::nt) __builtin_nontemporal_store(h, &at<S>(f, idx));
+  if constexpr (NT) __builtin_nontemporal_store(h, &at<S>(f, idx));
   else at<S>(f, idx) = h;
   if constexpr (P::comp) {
     if (comp_f) {
@@ -138,26 +140,21 @@ __device__ __forceinline__ typename P::A load_rt(const AT &a, int f, u32 idx)
 {
   return load_raw<P>(a.f[f], a.lo[f], (P::cmask >> f) & 1u, idx);
 }
-template <class P, class AT>
-__device__ __forceinline__ void store_rt(const AT &a, int f, u32 idx, typename P::A x)
-{
-  store_raw<P>(a.f[f], a.lo[f], (P::cmask >> f) & 1u, idx, x);
-}
 // compile-time field index: the residual test folds away
 template <class P, int F, class AT>
 __device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
 {
   return load_rt<P>(a, F, idx);
 }
-template <class P, int F, class AT>
+template <class P, int F, bool NT, class AT>
 __device__ __forceinline__ typename P::A LO(const AT &a, u32 idx)
 {
-  return load_own<P>(a.f[F], a.lo[F], (P::cmask >> F) & 1u, idx);
+  return load_own<P, NT>(a.f[F], a.lo[F], (P::cmask >> F) & 1u, idx);
 }
-template <class P, int F, class AT>
+template <class P, int F, bool NT, class AT>
 __device__ __forceinline__ void ST(const AT &a, u32 idx, typename P::A x)
 {
-  store_rt<P>(a, F, idx, x);
+  store_raw<P, NT>(a.f[F], a.lo[F], (P::cmask >> F) & 1u, idx, x);
 }
 
 // staggered difference across the centre of a 2h-point window w[0..2h-1]
@@ -353,8 +350,8 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     A z3 = dbw<P, FTYZ, ORDER>(a, c, sy, tyz_c);
     A s3 = dwin<ORDER>(wp);
 
-    const A qx0 = LO<P, FQX>(a, c), qy0 = LO<P, FQY>(a, c), qz0 = LO<P, FQZ>(a, c);
-    const A vx0 = LO<P, FVX>(a, c), vy0 = LO<P, FVY>(a, c), vz0 = LO<P, FVZ>(a, c);
+    const A qx0 = LO<P, FQX, P::nt_v>(a, c), qy0 = LO<P, FQY, P::nt_v>(a, c), qz0 = LO<P, FQZ, P::nt_v>(a, c);
+    const A vx0 = LO<P, FVX, P::nt_v>(a, c), vy0 = LO<P, FVY, P::nt_v>(a, c), vz0 = LO<P, FVZ, P::nt_v>(a, c);
     A kv[14];  // K[0..13]: U1 xyz, U2 xyz, W1 xyz, W2 xyz, W3x, W3y
     coef_block<P, 14>(a, a.kv, c, 0, kv);
     const A w1x = kv[K_W1X], w2x = kv[K_W2X], w3x = kv[K_W3X];
@@ -413,16 +410,16 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     const A qx1 = w1x * qx0 - w2x * sx - w3x * s1;
     const A qy1 = w1y * qy0 - w2y * sy_ - w3y * s2;
     const A qz1 = w1z * qz0 - w2z * sz_ - w3y * s3;
-    ST<P, FQX>(a, c, qx1);
-    ST<P, FQY>(a, c, qy1);
-    ST<P, FQZ>(a, c, qz1);
+    ST<P, FQX, P::nt_v>(a, c, qx1);
+    ST<P, FQY, P::nt_v>(a, c, qy1);
+    ST<P, FQZ, P::nt_v>(a, c, qz1);
     // the solid update sees q as stored (rounded), as the reference re-reads it; q is
     // never a compensated field, so the stored value is the plain storage rounding
     using S = typename P::S;
     const A qx1s = (A)(S)qx1, qy1s = (A)(S)qy1, qz1s = (A)(S)qz1;
-    ST<P, FVX>(a, c, vx0 + u1x * sx - u2x * (qx1s - qx0));
-    ST<P, FVY>(a, c, vy0 + u1y * sy_ - u2y * (qy1s - qy0));
-    ST<P, FVZ>(a, c, vz0 + u1z * sz_ - u2z * (qz1s - qz0));
+    ST<P, FVX, P::nt_v>(a, c, vx0 + u1x * sx - u2x * (qx1s - qx0));
+    ST<P, FVY, P::nt_v>(a, c, vy0 + u1y * sy_ - u2y * (qy1s - qy0));
+    ST<P, FVZ, P::nt_v>(a, c, vz0 + u1z * sz_ - u2z * (qz1s - qz0));
 
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) {
@@ -505,9 +502,9 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     A uyz = dwin<ORDER>(wvy);
     A uzy = dfw<P, FVZ, ORDER>(a, c, sy, vz_c);
 
-    const A p0 = LO<P, FP>(a, c), txx0 = LO<P, FTXX>(a, c), tyy0 = LO<P, FTYY>(a, c);
-    const A tzz0 = LO<P, FTZZ>(a, c), txy0 = LO<P, FTXY>(a, c), tyz0 = LO<P, FTYZ>(a, c);
-    const A txz0 = LO<P, FTXZ>(a, c);
+    const A p0 = LO<P, FP, P::nt_t>(a, c), txx0 = LO<P, FTXX, P::nt_t>(a, c), tyy0 = LO<P, FTYY, P::nt_t>(a, c);
+    const A tzz0 = LO<P, FTZZ, P::nt_t>(a, c), txy0 = LO<P, FTXY, P::nt_t>(a, c), tyz0 = LO<P, FTYZ, P::nt_t>(a, c);
+    const A txz0 = LO<P, FTXZ, P::nt_t>(a, c);
     A kt[8];  // K[14..21]: P1, P2, S1, S2, S3, Sxy, Sxz, Syz
     coef_block<P, 8>(a, a.kt, c, K_P1, kt);
     const A s1 = kt[K_S1 - K_P1], s2 = kt[K_S2 - K_P1], s3 = kt[K_S3 - K_P1];
@@ -557,13 +554,13 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
 
     // pore pressure and stresses (kernel.cu:484-490)
     const A divw = wx + wy + wz;
-    ST<P, FP>(a, c, p0 - kp1 * (uxx + uyy + uzz) - kp2 * divw);
-    ST<P, FTXX>(a, c, txx0 + s1 * (uyy + uzz) + s2 * uxx + s3 * divw);
-    ST<P, FTYY>(a, c, tyy0 + s1 * (uxx + uzz) + s2 * uyy + s3 * divw);
-    ST<P, FTZZ>(a, c, tzz0 + s1 * (uxx + uyy) + s2 * uzz + s3 * divw);
-    ST<P, FTXY>(a, c, txy0 + kxy * (uxy + uyx));
-    ST<P, FTYZ>(a, c, tyz0 + kyz * (uyz + uzy));
-    ST<P, FTXZ>(a, c, txz0 + kxz * (uxz + uzx));
+    ST<P, FP, P::nt_t>(a, c, p0 - kp1 * (uxx + uyy + uzz) - kp2 * divw);
+    ST<P, FTXX, P::nt_t>(a, c, txx0 + s1 * (uyy + uzz) + s2 * uxx + s3 * divw);
+    ST<P, FTYY, P::nt_t>(a, c, tyy0 + s1 * (uxx + uzz) + s2 * uyy + s3 * divw);
+    ST<P, FTZZ, P::nt_t>(a, c, tzz0 + s1 * (uxx + uyy) + s2 * uzz + s3 * divw);
+    ST<P, FTXY, P::nt_t>(a, c, txy0 + kxy * (uxy + uyx));
+    ST<P, FTYZ, P::nt_t>(a, c, tyz0 + kyz * (uyz + uzy));
+    ST<P, FTXZ, P::nt_t>(a, c, txz0 + kxz * (uxz + uzx));
 
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) {
