@@ -260,8 +260,9 @@ def stream_copy_rate(mib=1024):
 def hip_copy_rates(mib=1024):
     """HBM rate (GB/s, bytes read + written) of the library's streaming kernels, 20 timed
     launches each: a 16-byte copy of `mib` MiB with plain and with nontemporal accesses, and the
-    FP32 velocity kernel's access pattern without its stencil (the 256^3 cells: 13 fields and
-    the packed coefficient block read, 6 fields written) with plain and nontemporal stores."""
+    velocity kernel's access pattern without its stencil (the 256^3 cells: 13 fields and the
+    packed coefficient block read, 6 fields written), FP32 and FP16, with plain and
+    nontemporal stores."""
     import ctypes as C
 
     from biot_amd import _lib
@@ -269,7 +270,9 @@ def hip_copy_rates(mib=1024):
     out = {}
     for name, variant, nbytes in (("plain", 0, mib * 2 ** 20), ("nontemporal", 1, mib * 2 ** 20),
                                   ("velocity_pattern", 2, 4 * 256 ** 3),
-                                  ("velocity_pattern_nt", 3, 4 * 256 ** 3)):
+                                  ("velocity_pattern_nt", 3, 4 * 256 ** 3),
+                                  ("velocity_pattern_fp16", 4, 4 * 256 ** 3),
+                                  ("velocity_pattern_fp16_nt", 5, 4 * 256 ** 3)):
         g = C.c_double()
         rc = _lib.lib().biot_copy_bandwidth(0, nbytes, 20, variant, C.byref(g))
         out[name] = round(g.value, 1) if rc == 0 else None
@@ -333,7 +336,8 @@ def main():
     hip_copy = hip_copy_rates()
     log(f"[bench] plain copies: torch {copy_gbs} GB/s, HIP 16-byte {hip_copy['plain']} GB/s, "
         f"nontemporal {hip_copy['nontemporal']} GB/s; velocity access pattern without stencil "
-        f"{hip_copy['velocity_pattern']} GB/s, nontemporal stores {hip_copy['velocity_pattern_nt']} GB/s")
+        f"{hip_copy['velocity_pattern']} GB/s, nontemporal stores {hip_copy['velocity_pattern_nt']} GB/s; "
+        f"FP16 pattern {hip_copy['velocity_pattern_fp16']} / {hip_copy['velocity_pattern_fp16_nt']} GB/s")
     cfg = configs.layered256(order=4, nt=args.warmup + args.steps, n=args.size)
     n = cfg["nx"] * cfg["ny"] * cfg["nz"]
     r32 = run_precision(cfg, "fp32", args.steps, args.warmup)

@@ -319,7 +319,8 @@ int biot_device_bytes(biot_handle *h, int64_t *bytes);
  * nontemporal (1) accesses.  Variants 2 and 3: the FP32 velocity kernel's access pattern
  * without its stencil, over bytes/4 cells (bytes at most 1 GiB): 13 float fields and a
  * 56-byte packed block read, 6 float fields written per cell (132 B), ordinary (2) or
- * nontemporal (3) stores. */
+ * nontemporal (3) stores.  Variants 4 and 5: the FP16 velocity kernel's pattern over bytes/4
+ * cells: 13 binary16 fields and a 28-byte block read, 6 binary16 fields written (66 B). */
 int biot_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double *GBs);
 
 #ifdef __cplusplus

@@ -9,7 +9,7 @@ from biot_amd import _lib
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_copy_bandwidth_rate(variant):
     g = C.c_double()
     assert _lib.lib().biot_copy_bandwidth(0, 256 * 2 ** 20, 5, variant, C.byref(g)) == 0
@@ -19,7 +19,7 @@ def test_copy_bandwidth_rate(variant):
 
 def test_copy_bandwidth_arguments():
     L, g = _lib.lib(), C.c_double()
-    for args in ((0, 100, 5, 0), (0, 0, 5, 0), (0, 1 << 20, 0, 0), (0, 1 << 20, 5, 4), (-1, 1 << 20, 5, 0),
+    for args in ((0, 100, 5, 0), (0, 0, 5, 0), (0, 1 << 20, 0, 0), (0, 1 << 20, 5, 6), (-1, 1 << 20, 5, 0),
                  (0, (1 << 30) + 16, 5, 2)):
         assert L.biot_copy_bandwidth(*args, C.byref(g)) != 0
     assert L.biot_copy_bandwidth(0, 1 << 20, 5, 0, None) != 0
