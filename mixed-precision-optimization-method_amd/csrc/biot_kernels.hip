@@ -276,7 +276,9 @@ __device__ __forceinline__ A prof(const void *p, int i)
 enum { FVX = 0, FVY, FVZ, FQX, FQY, FQZ, FTXX, FTYY, FTZZ, FTXY, FTXZ, FTYZ, FP };
 
 // one x-y tile of the update box, planes kb..ke-1 of z chunk tz
-template <class P, int ORDER, class AT>
+// NT: the launch allows the policy's nontemporal accesses (the single-grid kernels; the
+// batched kernels, whose shot windows are reused from the caches, do not)
+template <class P, int ORDER, bool NT, class AT>
 __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int tz)
 {
   using A = typename P::A;
@@ -350,8 +352,8 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     A z3 = dbw<P, FTYZ, ORDER>(a, c, sy, tyz_c);
     A s3 = dwin<ORDER>(wp);
 
-    const A qx0 = LO<P, FQX, P::nt_v>(a, c), qy0 = LO<P, FQY, P::nt_v>(a, c), qz0 = LO<P, FQZ, P::nt_v>(a, c);
-    const A vx0 = LO<P, FVX, P::nt_v>(a, c), vy0 = LO<P, FVY, P::nt_v>(a, c), vz0 = LO<P, FVZ, P::nt_v>(a, c);
+    const A qx0 = LO<P, FQX, (P::nt_v && NT)>(a, c), qy0 = LO<P, FQY, (P::nt_v && NT)>(a, c), qz0 = LO<P, FQZ, (P::nt_v && NT)>(a, c);
+    const A vx0 = LO<P, FVX, (P::nt_v && NT)>(a, c), vy0 = LO<P, FVY, (P::nt_v && NT)>(a, c), vz0 = LO<P, FVZ, (P::nt_v && NT)>(a, c);
     A kv[14];  // K[0..13]: U1 xyz, U2 xyz, W1 xyz, W2 xyz, W3x, W3y
     coef_block<P, 14>(a, a.kv, c, 0, kv);
     const A w1x = kv[K_W1X], w2x = kv[K_W2X], w3x = kv[K_W3X];
@@ -410,16 +412,16 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     const A qx1 = w1x * qx0 - w2x * sx - w3x * s1;
     const A qy1 = w1y * qy0 - w2y * sy_ - w3y * s2;
     const A qz1 = w1z * qz0 - w2z * sz_ - w3y * s3;
-    ST<P, FQX, P::nt_v>(a, c, qx1);
-    ST<P, FQY, P::nt_v>(a, c, qy1);
-    ST<P, FQZ, P::nt_v>(a, c, qz1);
+    ST<P, FQX, (P::nt_v && NT)>(a, c, qx1);
+    ST<P, FQY, (P::nt_v && NT)>(a, c, qy1);
+    ST<P, FQZ, (P::nt_v && NT)>(a, c, qz1);
     // the solid update sees q as stored (rounded), as the reference re-reads it; q is
     // never a compensated field, so the stored value is the plain storage rounding
     using S = typename P::S;
     const A qx1s = (A)(S)qx1, qy1s = (A)(S)qy1, qz1s = (A)(S)qz1;
-    ST<P, FVX, P::nt_v>(a, c, vx0 + u1x * sx - u2x * (qx1s - qx0));
-    ST<P, FVY, P::nt_v>(a, c, vy0 + u1y * sy_ - u2y * (qy1s - qy0));
-    ST<P, FVZ, P::nt_v>(a, c, vz0 + u1z * sz_ - u2z * (qz1s - qz0));
+    ST<P, FVX, (P::nt_v && NT)>(a, c, vx0 + u1x * sx - u2x * (qx1s - qx0));
+    ST<P, FVY, (P::nt_v && NT)>(a, c, vy0 + u1y * sy_ - u2y * (qy1s - qy0));
+    ST<P, FVZ, (P::nt_v && NT)>(a, c, vz0 + u1z * sz_ - u2z * (qz1s - qz0));
 
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) {
@@ -432,7 +434,7 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
 }
 
 // one x-y tile of the update box, planes kb..ke-1 of z chunk tz
-template <class P, int ORDER, class AT>
+template <class P, int ORDER, bool NT, class AT>
 __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
 {
   using A = typename P::A;
@@ -502,9 +504,9 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     A uyz = dwin<ORDER>(wvy);
     A uzy = dfw<P, FVZ, ORDER>(a, c, sy, vz_c);
 
-    const A p0 = LO<P, FP, P::nt_t>(a, c), txx0 = LO<P, FTXX, P::nt_t>(a, c), tyy0 = LO<P, FTYY, P::nt_t>(a, c);
-    const A tzz0 = LO<P, FTZZ, P::nt_t>(a, c), txy0 = LO<P, FTXY, P::nt_t>(a, c), tyz0 = LO<P, FTYZ, P::nt_t>(a, c);
-    const A txz0 = LO<P, FTXZ, P::nt_t>(a, c);
+    const A p0 = LO<P, FP, (P::nt_t && NT)>(a, c), txx0 = LO<P, FTXX, (P::nt_t && NT)>(a, c), tyy0 = LO<P, FTYY, (P::nt_t && NT)>(a, c);
+    const A tzz0 = LO<P, FTZZ, (P::nt_t && NT)>(a, c), txy0 = LO<P, FTXY, (P::nt_t && NT)>(a, c), tyz0 = LO<P, FTYZ, (P::nt_t && NT)>(a, c);
+    const A txz0 = LO<P, FTXZ, (P::nt_t && NT)>(a, c);
     A kt[8];  // K[14..21]: P1, P2, S1, S2, S3, Sxy, Sxz, Syz
     coef_block<P, 8>(a, a.kt, c, K_P1, kt);
     const A s1 = kt[K_S1 - K_P1], s2 = kt[K_S2 - K_P1], s3 = kt[K_S3 - K_P1];
@@ -554,13 +556,13 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
 
     // pore pressure and stresses (kernel.cu:484-490)
     const A divw = wx + wy + wz;
-    ST<P, FP, P::nt_t>(a, c, p0 - kp1 * (uxx + uyy + uzz) - kp2 * divw);
-    ST<P, FTXX, P::nt_t>(a, c, txx0 + s1 * (uyy + uzz) + s2 * uxx + s3 * divw);
-    ST<P, FTYY, P::nt_t>(a, c, tyy0 + s1 * (uxx + uzz) + s2 * uyy + s3 * divw);
-    ST<P, FTZZ, P::nt_t>(a, c, tzz0 + s1 * (uxx + uyy) + s2 * uzz + s3 * divw);
-    ST<P, FTXY, P::nt_t>(a, c, txy0 + kxy * (uxy + uyx));
-    ST<P, FTYZ, P::nt_t>(a, c, tyz0 + kyz * (uyz + uzy));
-    ST<P, FTXZ, P::nt_t>(a, c, txz0 + kxz * (uxz + uzx));
+    ST<P, FP, (P::nt_t && NT)>(a, c, p0 - kp1 * (uxx + uyy + uzz) - kp2 * divw);
+    ST<P, FTXX, (P::nt_t && NT)>(a, c, txx0 + s1 * (uyy + uzz) + s2 * uxx + s3 * divw);
+    ST<P, FTYY, (P::nt_t && NT)>(a, c, tyy0 + s1 * (uxx + uzz) + s2 * uyy + s3 * divw);
+    ST<P, FTZZ, (P::nt_t && NT)>(a, c, tzz0 + s1 * (uxx + uyy) + s2 * uzz + s3 * divw);
+    ST<P, FTXY, (P::nt_t && NT)>(a, c, txy0 + kxy * (uxy + uyx));
+    ST<P, FTYZ, (P::nt_t && NT)>(a, c, tyz0 + kyz * (uyz + uzy));
+    ST<P, FTXZ, (P::nt_t && NT)>(a, c, txz0 + kxz * (uxz + uzx));
 
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) {
@@ -577,14 +579,14 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs a)
 {
   int tx, ty, tz;
   if (!tile_of(a, tx, ty, tz)) return;
-  velocity_tile<P, ORDER>(a, tx, ty, tz);
+  velocity_tile<P, ORDER, true>(a, tx, ty, tz);
 }
 template <class P, int ORDER>
 __global__ void __launch_bounds__(256) k_stress(const StepArgs a)
 {
   int tx, ty, tz;
   if (!tile_of(a, tx, ty, tz)) return;
-  stress_tile<P, ORDER>(a, tx, ty, tz);
+  stress_tile<P, ORDER, true>(a, tx, ty, tz);
 }
 
 // Batched forms: one persistent grid walks the tiles of every shot.  The shots' tile
@@ -618,8 +620,8 @@ __global__ void __launch_bounds__(256) k_step_batch(const BatchArgs b)
     const CStepArgs &a = ((CStepArgs *)b.shots)[s];
     const int lt = t - ((CInt *)b.tile_start)[s];
     const int tx = lt % a.gx, r = lt / a.gx, ty = r % a.gy, tz = r / a.gy;
-    if (VEL) velocity_tile<P, ORDER>(a, tx, ty, tz);
-    else stress_tile<P, ORDER>(a, tx, ty, tz);
+    if (VEL) velocity_tile<P, ORDER, false>(a, tx, ty, tz);
+    else stress_tile<P, ORDER, false>(a, tx, ty, tz);
   }
 }
 
