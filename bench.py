@@ -28,9 +28,10 @@ def log(*a):
 
 
 def slab_counts(n, L, R, h):
-    """Updated cells (h..n-1-h) that lie in the low/high absorbing slab of one axis."""
+    """Updated cells (h..n-1-h) that lie in the low or the high absorbing slab of one axis
+    (a cell in both counts once, as the kernels treat it: in the low slab)."""
     lo = max(0, min(L, n - h) - h)
-    hi = max(0, (n - h) - max(n - R, h))
+    hi = max(0, (n - h) - max(n - R, h, L))
     return lo + hi
 
 
