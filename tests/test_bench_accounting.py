@@ -52,3 +52,14 @@ def test_fp32_step_bytes_per_interior_cell():
     """Without absorbing slabs the fused FP32 step moves 26 x 4 + 22 x 4 = 192 B per cell."""
     v, t, step, upd, memb = bench.algorithmic_bytes(16, 16, 16, 2, (0, 0, 0, 0, 0, 0), "fp32")
     assert memb == 0 and step == 192 * upd and v == 132 * upd and t == 112 * upd
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The benchmark measures one GPU; asked for more it exits with status 2 and says so,
+    before it touches a device."""
+    import subprocess
+
+    r = subprocess.run([sys.executable, bench.__file__, "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and not r.stdout.strip()
