@@ -153,10 +153,12 @@ def test_half_precision_full_size_perturbed512():
     ref = traces["fp64"]
     surf = {p: [rel_l2(traces[p][ci[f], :, :nsurf], ref[ci[f], :, :nsurf]) for f in ("vx", "vz", "p")]
             for p in ("fp16c", "fp16t", "fp16")}
-    col = {p: rel_l2(traces[p][ci["txx"], :, nsurf:], ref[ci["txx"], :, nsurf:]) for p in surf}
+    cols = {p: {f: rel_l2(traces[p][ci[f], :, nsurf:], ref[ci[f], :, nsurf:]) for f in fields} for p in surf}
+    col = {p: cols[p]["txx"] for p in surf}
     for p in surf:
         print(f"  {p} surface traces rel L2 vs fp64 (vx, vz, p) = " + " ".join(f"{x:.3e}" for x in surf[p]) +
-              f"; source-column txx {col[p]:.3e}")
+              "; source column (txx, vx, vz, p) = " +
+              " ".join(f"{cols[p][f]:.3e}" for f in ("txx", "vx", "vz", "p")))
     assert all(np.isfinite(traces[p]).all() for p in ("fp16c", "fp16t", "fp16"))
     for p in ("fp16c", "fp16t"):
         assert max(surf[p]) <= 1e-2 and col[p] <= 1e-2
