@@ -123,6 +123,24 @@ def run_precision(cfg, precision, steps, warmup):
                 finite=finite, shape=list(shape))
 
 
+def host_cpu():
+    """The host the CPU baseline ran on: logical CPUs (`nproc` of the machine), the CPUs this
+    process may run on, and the processor model from /proc/cpuinfo (None where absent)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return dict(nproc=os.cpu_count(), usable_cpus=usable, cpu_model=model)
+
+
 def cpu_baseline(cfg, nsteps):
     """The oracle (the reference update arithmetic restated in C, OpenMP over z-planes)
     timed on the host cores over a bounded number of steps of the same workload."""
@@ -146,7 +164,7 @@ def cpu_baseline(cfg, nsteps):
     return dict(value=n * nsteps / dt / 1e6, unit="Mcell-updates/s", cores=oracle.num_threads(),
                 kind="port",
                 sample=f"{nsteps} steps of the same 256^3 order-4 FP32 workload, oracle in C "
-                       f"(-O3, OpenMP), {dt:.1f} s")
+                       f"(-O3, OpenMP), {dt:.1f} s", host=host_cpu())
 
 
 def adjoint_shot(repeats=2):

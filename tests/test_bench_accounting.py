@@ -63,3 +63,10 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and not r.stdout.strip()
+
+
+def test_host_cpu_record():
+    h = bench.host_cpu()
+    assert set(h) == {"nproc", "usable_cpus", "cpu_model"}
+    assert h["nproc"] >= 1 and 1 <= h["usable_cpus"] <= h["nproc"]
+    assert h["cpu_model"] is None or isinstance(h["cpu_model"], str)
