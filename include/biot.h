@@ -13,7 +13,8 @@
  * Conventions
  *   - Every call returns BIOT_OK (0) or a BIOT_ERR_* code; nothing exits or aborts
  *     (the reference's CUDA_CHECK -> exit(EXIT_FAILURE), kernel.cu:42-52, is replaced).
- *     biot_last_error() gives the message of the last failure on a handle.
+ *     biot_last_error(h) gives the message of the last failure on a handle;
+ *     biot_last_error(NULL) that of the calling thread's last failed biot_create.
  *   - Arrays are host pointers, x fastest: cell (i, j, k) is i + j*nx + k*nx*ny
  *     (main.cpp:1489).  Values are in the reference representation: velocities scaled
  *     by Cv, stresses and pressure by Cp (main.cpp:248-249).
@@ -97,6 +98,10 @@ typedef struct biot_medium {
 typedef struct biot_handle biot_handle;
 
 /* ---- lifetime ---- */
+/* Sizes: the kernels address each array with 32-bit byte offsets from its base, so a grid
+ * is refused (BIOT_ERR_ARG) when one field array (n cells x the storage width) or one axis's
+ * C-PML memory (8 carries x the slab's cells x the carry width) would exceed 4 GiB; the
+ * adjoint refuses a handle whose multiplier arrays would (n x 4 bytes, x 8 on FP64). */
 int biot_create(const biot_grid *grid, biot_handle **out);
 int biot_destroy(biot_handle *h);
 const char *biot_last_error(const biot_handle *h);

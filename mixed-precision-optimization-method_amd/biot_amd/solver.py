@@ -32,7 +32,9 @@ class Solver:
         h = C.c_void_p()
         rc = lib().biot_create(C.byref(g), C.byref(h))
         if rc != BIOT_OK:
-            raise BiotError(rc, "biot_create failed for grid %dx%dx%d order %d" % (nx, ny, nz, order))
+            msg = lib().biot_last_error(None)
+            raise BiotError(rc, "biot_create failed for grid %dx%dx%d order %d: %s"
+                            % (nx, ny, nz, order, msg.decode() if msg else ""))
         self._h = h
         self.ncomp = 0
         self.nrec = 0

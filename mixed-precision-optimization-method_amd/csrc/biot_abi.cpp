@@ -112,9 +112,12 @@ extern "C" const char *biot_status_string(int s)
   }
 }
 
+// message of the calling thread's last failed biot_create (biot_last_error(NULL))
+static thread_local std::string t_create_err;
+
 extern "C" const char *biot_last_error(const biot_handle *h)
 {
-  return h ? h->err.c_str() : "null handle";
+  return h ? h->err.c_str() : t_create_err.c_str();
 }
 
 static int validate_grid(const biot_grid *g, std::string &why)
@@ -157,11 +160,17 @@ static int validate_grid(const biot_grid *g, std::string &why)
 
 extern "C" int biot_create(const biot_grid *g, biot_handle **out)
 {
-  if (!g || !out) return BIOT_ERR_ARG;
+  t_create_err.clear();
+  auto refuse = [](biot_handle *h, int rc, const char *why) {
+    delete h;
+    t_create_err = why;
+    return rc;
+  };
+  if (!g || !out) return refuse(nullptr, BIOT_ERR_ARG, "grid and output pointers are required");
   *out = nullptr;
   std::string why;
   int rc = validate_grid(g, why);
-  if (rc) return rc;
+  if (rc) return refuse(nullptr, rc, why.c_str());
   biot_handle *h = new biot_handle();
   h->g = *g;
   h->prec = g->precision;
@@ -170,21 +179,6 @@ extern "C" int biot_create(const biot_grid *g, biot_handle **out)
   h->n = (int64_t)g->nx * g->ny * g->nz;
   for (int k = 0; k < NKC; ++k) h->ks[k] = 1.0f;
   for (int f = 0; f < NFIELD; ++f) h->fscale[f] = 1.0f;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-    delete h;
-    return BIOT_ERR_HIP;
-  }
-  if (g->device < 0 || g->device >= ndev) {
-    delete h;
-    return BIOT_ERR_ARG;
-  }
-  if (hipSetDevice(g->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess ||
-      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
-    delete h;
-    return BIOT_ERR_HIP;
-  }
-  h->shape = default_shape(h->prec, g->order);
   // PML profiles (main.cpp:537-566) and slab widths
   const int dims[3] = {g->nx, g->ny, g->nz};
   for (int ax = 0; ax < 3; ++ax) {
@@ -209,22 +203,27 @@ extern "C" int biot_create(const biot_grid *g, biot_handle **out)
   h->slab[2] = (int64_t)g->nx * g->ny * (h->L[2] + h->R[2]);
   // the kernels address every array with 32-bit byte offsets from its base
   const int64_t lim = (int64_t)1 << 32;
-  bool fits = h->n * (int64_t)h->esz < lim;
-  for (int ax = 0; ax < 3; ++ax) fits = fits && NCARRY * h->slab[ax] * (int64_t)h->csz < lim;
-  if (!fits) {
-    delete h;
-    return BIOT_ERR_ARG;
-  }
+  if (h->n * (int64_t)h->esz >= lim)
+    return refuse(h, BIOT_ERR_ARG, "grid too large: one field array would exceed 4 GiB");
+  for (int ax = 0; ax < 3; ++ax)
+    if (NCARRY * h->slab[ax] * (int64_t)h->csz >= lim)
+      return refuse(h, BIOT_ERR_ARG, "absorbing slabs too large: one axis's C-PML memory would exceed 4 GiB");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return refuse(h, BIOT_ERR_HIP, "no HIP device");
+  if (g->device < 0 || g->device >= ndev) return refuse(h, BIOT_ERR_ARG, "device index out of range");
+  if (hipSetDevice(g->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess ||
+      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return refuse(h, BIOT_ERR_HIP, "device, stream or event set-up failed");
+  h->shape = default_shape(h->prec, g->order);
   rc = h->allocate();
   if (rc) {
-    std::string e = h->err;
-    delete h;
-    return rc;
+    const std::string e = h->err;
+    return refuse(h, rc, e.c_str());
   }
   rc = h->upload_profiles();
   if (rc) {
-    delete h;
-    return rc;
+    const std::string e = h->err.empty() ? std::string("PML profile upload failed") : h->err;
+    return refuse(h, rc, e.c_str());
   }
   *out = h;
   return BIOT_OK;

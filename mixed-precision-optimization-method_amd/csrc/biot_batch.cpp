@@ -212,7 +212,7 @@ extern "C" int biot_batch_add_shot(biot_batch *b, const biot_shot *s, const biot
   g.device = c.device;
   biot_handle *h = nullptr;
   int rc = biot_create(&g, &h);
-  if (rc) return bfail(b, rc, "shot %zu: window grid rejected", b->shots.size());
+  if (rc) return bfail(b, rc, "shot %zu: window grid rejected: %s", b->shots.size(), biot_last_error(nullptr));
   if ((rc = biot_set_params(h, p)) || (rc = biot_set_source(h, s->src_cell, s->src_mask, wavelet, nt)) ||
       (rc = biot_set_receivers(h, s->rec_cells, s->nrec, s->rec_mask, nt))) {
     const int r2 = hfail(b, h, rc, "shot set-up");

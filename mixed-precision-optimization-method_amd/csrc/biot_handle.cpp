@@ -84,6 +84,15 @@ size_t biot_handle::lam_bytes() const
 int biot_handle::alloc_adjoint(int every)
 {
   const int64_t m = (int64_t)ncomp * nt_max * nrec;
+  // the adjoint kernels address each multiplier, derivative and carry-multiplier array with
+  // 32-bit byte offsets from its base, as the forward kernels do their arrays
+  const int64_t asz = prec == BIOT_FP64 ? 8 : 4, lim = (int64_t)1 << 32;
+  bool fits = n * asz < lim;
+  for (int ax = 0; ax < 3; ++ax) fits = fits && NCARRY * slab[ax] * asz < lim;
+  if (!fits) {
+    err = "grid too large for the adjoint (an adjoint array would exceed 4 GiB)";
+    return BIOT_ERR_ARG;
+  }
   auto get = [&](void **p, size_t bytes) -> bool {
     if (*p) return true;
     if (hipMalloc(p, bytes) != hipSuccess) {

@@ -11,6 +11,7 @@ import pytest
 import oracle
 from biot_amd import _lib, media
 from biot_amd._lib import BIOT_OK, Grid
+from biot_amd.solver import BiotError, Solver
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -124,3 +125,48 @@ def test_create_without_device_fails_loudly(precision):
         L.biot_destroy(h)
         pytest.skip("a GPU is present")
     assert rc == 2  # BIOT_ERR_HIP
+
+
+def _create_refusal(nx, ny, nz, precision, npml, device=0):
+    L = _lib.lib()
+    h = C.c_void_p()
+    g = Grid(nx=nx, ny=ny, nz=nz, order=4, precision=precision, npml=npml, dx=5, dt=4e-4, pml_v0=4000,
+             pml_R=1e-6, Cv=1e8, Cp=1, device=device)
+    rc = L.biot_create(C.byref(g), C.byref(h))
+    assert rc != BIOT_OK and not h.value
+    return rc, L.biot_last_error(None).decode()
+
+
+def test_create_refuses_arrays_beyond_32bit_offsets():
+    """The kernels form byte offsets in 32 bits from each array's base: a grid whose field
+    array (n x storage width) or one axis's C-PML memory (8 carries x slab cells x carry width)
+    reaches 4 GiB is refused with its reason before any device call, so the result does not
+    depend on a GPU being present."""
+    # FP64 fields: 1024*1024*512 cells * 8 B = 4 GiB exactly
+    assert _create_refusal(1024, 1024, 512, 2, 10) == (1, "grid too large: one field array would exceed 4 GiB")
+    # FP32 fields: 2^30 cells * 4 B
+    assert _create_refusal(1024, 1024, 1024, 0, 10)[1].startswith("grid too large")
+    # FP32 1000^3 (fields 4.0e9 B) with a 100-cell PML: each axis's carries 8 * 2.0e8 * 4 B
+    rc, msg = _create_refusal(1000, 1000, 1000, 0, 100)
+    assert rc == 1 and msg.startswith("absorbing slabs too large")
+    # FP16 carries are binary16: the same slabs fit (8 * 2.0e8 * 2 B = 3.2e9 B)
+    rc, msg = _create_refusal(1000, 1000, 1000, 1, 100, device=-1)
+    assert msg in ("no HIP device", "device index out of range")
+    # FP16C keeps float carries: refused again
+    assert _create_refusal(1000, 1000, 1000, 3, 100)[1].startswith("absorbing slabs too large")
+    # just inside both limits the checks pass and only the device index stops it
+    rc, msg = _create_refusal(1000, 1000, 1000, 0, 10, device=-1)
+    assert msg in ("no HIP device", "device index out of range")
+
+
+def test_create_reports_argument_errors_through_null_handle():
+    rc, msg = _create_refusal(64, 64, 64, 0, 40)
+    assert rc == 1 and msg == "npml out of range for the grid"
+    L = _lib.lib()
+    g = Grid(nx=64, ny=64, nz=64, order=3, precision=0, npml=10, dx=5, dt=4e-4, pml_v0=4000,
+             pml_R=1e-6, Cv=1e8, Cp=1, device=0)
+    h = C.c_void_p()
+    assert L.biot_create(C.byref(g), C.byref(h)) == 1
+    assert L.biot_last_error(None) == b"order must be 2 or 4, got 3"
+    with pytest.raises(BiotError, match="npml out of range"):
+        Solver(64, 64, 64, 5.0, 4e-4, order=4, npml=40)
