@@ -24,6 +24,99 @@ def build():
     return LIB_PATH
 
 
+REF_SRC = "/root/reference"
+REF_LIB_PATH = os.path.join(HERE, "_ref", "liboutputfunc.so")
+
+
+def build_ref():
+    """Compile the reference's own outputfunc.cpp into _ref/liboutputfunc.so (`make ref`)
+    where the reference sources are present; returns the library path or None."""
+    if not os.path.isfile(os.path.join(REF_SRC, "outputfunc.cpp")):
+        return None
+    subprocess.run(["make", "-C", HERE, "-s", "ref", f"REF={REF_SRC}"], check=True)
+    return REF_LIB_PATH
+
+
+class RefOutputfunc:
+    """ctypes binding of the reference's compiled outputfunc.cpp (C++ linkage, so the symbols
+    carry their mangled names): the float*** allocator space3d, create_model_all and the text
+    writers wfile / wfile3d / wfile1d (outputfunc.cpp:14-151, 153-222)."""
+
+    F1 = C.POINTER(C.c_float)
+    F2 = C.POINTER(F1)
+    F3 = C.POINTER(F2)
+    # create_model_all's 14 volumes in argument order (outputfunc.cpp:153-154)
+    MODEL = ("vp", "vs", "rhos", "vf", "vfg", "rho", "rhof", "M", "C", "C1", "C2", "HH", "H2u", "mu")
+
+    def __init__(self, path=REF_LIB_PATH):
+        L = C.CDLL(path)
+        i = C.c_int
+        self.space2d = L["_Z7space2dii"]
+        self.space2d.argtypes, self.space2d.restype = [i, i], self.F2
+        self.free_space2d = L["_Z12free_space2dPPfi"]
+        self.free_space2d.argtypes, self.free_space2d.restype = [self.F2, i], None
+        self.space3d = L["_Z7space3diii"]
+        self.space3d.argtypes, self.space3d.restype = [i, i, i], self.F3
+        self.free_space3d = L["_Z12free_space3dPPPfii"]
+        self.free_space3d.argtypes, self.free_space3d.restype = [self.F3, i, i], None
+        self.create_model_all = L["_Z16create_model_allPPPfS1_S1_S1_S1_S1_S1_S1_S1_S1_S1_S1_S1_S1_iii"]
+        self.create_model_all.argtypes, self.create_model_all.restype = [self.F3] * 14 + [i, i, i], None
+        self.wfile = L["_Z5wfilePcPPfii"]
+        self.wfile.argtypes, self.wfile.restype = [C.c_char_p, self.F2, i, i], None
+        self.wfile3d = L["_Z7wfile3dPcPPPfiii"]
+        self.wfile3d.argtypes, self.wfile3d.restype = [C.c_char_p, self.F3, i, i, i], None
+        self.wfile1d = L["_Z7wfile1dPcPfiii"]
+        self.wfile1d.argtypes, self.wfile1d.restype = [C.c_char_p, self.F1, i, i, i], None
+
+    def model(self, nz, ny, nx):
+        """create_model_all on an nz x ny x nx grid: dict of (nz, ny, nx) float32 arrays."""
+        vols = {k: self.space3d(nz, ny, nx) for k in self.MODEL}
+        try:
+            self.create_model_all(*[vols[k] for k in self.MODEL], nz, ny, nx)
+            return {k: self.to_numpy(v, nz, ny, nx) for k, v in vols.items()}
+        finally:
+            for v in vols.values():
+                self.free_space3d(v, nz, ny)
+
+    @staticmethod
+    def to_numpy(a, nz, ny, nx):
+        out = np.empty((nz, ny, nx), np.float32)
+        for k in range(nz):
+            for j in range(ny):
+                C.memmove(out[k, j].ctypes.data, a[k][j], nx * 4)
+        return out
+
+    def write_2d(self, path, data):
+        """wfile on a space2d copy of a 2-D array."""
+        nr, nc = data.shape
+        a = self.space2d(nr, nc)
+        try:
+            for r in range(nr):
+                row = np.ascontiguousarray(data[r], np.float32)
+                C.memmove(a[r], row.ctypes.data, nc * 4)
+            self.wfile(os.fsencode(path), a, nr, nc)
+        finally:
+            self.free_space2d(a, nr)
+
+    def write_3d(self, path, vol):
+        """wfile3d on a space3d copy of an (nz, ny, nx) volume."""
+        nz, ny, nx = vol.shape
+        a = self.space3d(nz, ny, nx)
+        try:
+            for k in range(nz):
+                for j in range(ny):
+                    row = np.ascontiguousarray(vol[k, j], np.float32)
+                    C.memmove(a[k][j], row.ctypes.data, nx * 4)
+            self.wfile3d(os.fsencode(path), a, nz, ny, nx)
+        finally:
+            self.free_space3d(a, nz, ny)
+
+    def write_1d(self, path, vol):
+        """wfile1d on a flat (nz, ny, nx) volume."""
+        a = np.ascontiguousarray(vol, np.float32)
+        self.wfile1d(os.fsencode(path), a.ctypes.data_as(self.F1), *a.shape)
+
+
 class OrConfig(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("order", C.c_int),
                 ("mode", C.c_int), ("DT", C.c_float), ("Hinv", C.c_float),
