@@ -28,6 +28,12 @@
  * Space order 2 is the reference's stencil; order 4 (coefficients 9/8, -1/24) has no
  * reference arithmetic and is pinned by closed-form tests only.
  *
+ * Pins: every part above is checked by closed forms, invariants or brute force
+ * (tests/test_oracle_pins.py); the Biot moduli are in addition checked bit for bit against
+ * the reference's own outputfunc.cpp compiled from its sources (`make ref`, oracle/_ref).
+ * Parity unpinned: the rounding of the original nvcc-built binary (FMA contraction choices,
+ * cuda_fp16's double-to-half stores), which cannot be built or run here.
+ *
  * Field numbering (shared with the product's C-ABI): 0 vx 1 vy 2 vz 3 qx 4 qy 5 qz
  * 6 txx 7 tyy 8 tzz 9 txy 10 txz 11 tyz 12 p.
  */
