@@ -32,12 +32,19 @@ class BiotTraces(torch.autograd.Function):
         solver.forward_checkpointed(nt, every)
         d = solver.traces_f64() if solver.precision == "fp64" else solver.traces()
         ctx.solver = solver
+        ctx.generation = solver.ckpt_generation
         ctx.dtypes = [v.dtype for v in vals]
         ctx.device = lam.device
         return torch.from_numpy(d).to(lam.device)
 
     @staticmethod
     def backward(ctx, g):
+        # the adjoint recomputes from the solver's checkpoints, which belong to its latest
+        # checkpointed run: a later run on the same solver would give that run's gradient
+        if ctx.solver.ckpt_generation != ctx.generation:
+            raise RuntimeError("the solver ran another checkpointed forward after these traces were "
+                               "computed; call backward before reusing the solver, or use one solver "
+                               "per differentiable run")
         adj = g.detach().to("cpu", torch.float32).contiguous().numpy()
         grads, _ = ctx.solver.adjoint(adj, params=True)
         out = [torch.from_numpy(grads[k]).to(ctx.device, dt) for k, dt in zip(PARAM_NAMES, ctx.dtypes)]

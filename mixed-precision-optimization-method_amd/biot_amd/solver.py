@@ -40,6 +40,7 @@ class Solver:
         self.nrec = 0
         self.nt_max = 0
         self.rec_fields = []
+        self.ckpt_generation = 0  # counts checkpointed forward runs (the adjoint's state)
 
     # -- helpers -----------------------------------------------------------------------
     def _check(self, rc, what):
@@ -258,6 +259,7 @@ class Solver:
 
     def forward_checkpointed(self, nt, every=50):
         """Reset, nt recorded steps, a device checkpoint every `every` steps."""
+        self.ckpt_generation += 1
         self._check(lib().biot_forward_checkpointed(self._h, int(nt), int(every)),
                     "biot_forward_checkpointed")
 
