@@ -253,6 +253,8 @@ typedef struct biot_shot {
 } biot_shot;
 int biot_batch_create(const biot_batch_config *c, biot_batch **out);
 int biot_batch_destroy(biot_batch *b);
+/* the message of the last failure on a batch; with NULL, that of the calling thread's last
+ * failed biot_batch_create */
 const char *biot_batch_last_error(const biot_batch *b);
 /* window_params: the window's cells (its nx*ny*nz); receivers hold nt steps */
 int biot_batch_add_shot(biot_batch *b, const biot_shot *s, const biot_params *window_params,

@@ -23,7 +23,8 @@ class Batch:
         h = C.c_void_p()
         rc = lib().biot_batch_create(C.byref(c), C.byref(h))
         if rc != BIOT_OK:
-            raise BiotError(rc, "biot_batch_create")
+            msg = lib().biot_batch_last_error(None)
+            raise BiotError(rc, f"biot_batch_create: {msg.decode() if msg else ''}")
         self._h = h
         self.precision = precision
         self.shots = []  # (ncomp, nt, nrec, window shape, origin)

@@ -154,25 +154,35 @@ biot_batch::~biot_batch()
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+// message of the calling thread's last failed biot_batch_create (biot_batch_last_error(NULL))
+static thread_local std::string t_batch_create_err;
+
 extern "C" int biot_batch_create(const biot_batch_config *c, biot_batch **out)
 {
-  if (!c || !out) return BIOT_ERR_ARG;
+  t_batch_create_err.clear();
+  auto refuse = [](biot_batch *b, int rc, const char *why) {
+    delete b;
+    t_batch_create_err = why;
+    return rc;
+  };
+  if (!c || !out) return refuse(nullptr, BIOT_ERR_ARG, "config and output pointers are required");
   *out = nullptr;
-  if (c->nx < 8 || c->ny < 8 || c->nz < 8 || (c->order != 2 && c->order != 4) ||
-      (c->precision != BIOT_FP32 && c->precision != BIOT_FP16 && c->precision != BIOT_FP64) ||
-      !(c->dx > 0) || !(c->dt > 0) || c->npml < 0)
-    return BIOT_ERR_ARG;
+  if (c->nx < 8 || c->ny < 8 || c->nz < 8)
+    return refuse(nullptr, BIOT_ERR_ARG, "the model grid must be at least 8 cells on every axis");
+  if (c->order != 2 && c->order != 4) return refuse(nullptr, BIOT_ERR_ARG, "order must be 2 or 4");
+  if (c->precision != BIOT_FP32 && c->precision != BIOT_FP16 && c->precision != BIOT_FP64)
+    return refuse(nullptr, BIOT_ERR_ARG, "batches run BIOT_FP32, BIOT_FP16 or BIOT_FP64");
+  if (!(c->dx > 0) || !(c->dt > 0) || c->npml < 0)
+    return refuse(nullptr, BIOT_ERR_ARG, "dx and dt must be positive and npml >= 0");
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BIOT_ERR_HIP;
-  if (c->device < 0 || c->device >= ndev) return BIOT_ERR_ARG;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return refuse(nullptr, BIOT_ERR_HIP, "no HIP device");
+  if (c->device < 0 || c->device >= ndev) return refuse(nullptr, BIOT_ERR_ARG, "device index out of range");
   biot_batch *b = new biot_batch();
   b->cfg = *c;
   if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess ||
       hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-      hipMalloc((void **)&b->d_it, sizeof(int)) != hipSuccess) {
-    delete b;
-    return BIOT_ERR_HIP;
-  }
+      hipMalloc((void **)&b->d_it, sizeof(int)) != hipSuccess)
+    return refuse(b, BIOT_ERR_HIP, "device, stream, event or counter set-up failed");
   *out = b;
   return BIOT_OK;
 }
@@ -184,7 +194,10 @@ extern "C" int biot_batch_destroy(biot_batch *b)
   return BIOT_OK;
 }
 
-extern "C" const char *biot_batch_last_error(const biot_batch *b) { return b ? b->err.c_str() : ""; }
+extern "C" const char *biot_batch_last_error(const biot_batch *b)
+{
+  return b ? b->err.c_str() : t_batch_create_err.c_str();
+}
 
 extern "C" int biot_batch_add_shot(biot_batch *b, const biot_shot *s, const biot_params *p,
                                    const float *wavelet, int32_t nt)

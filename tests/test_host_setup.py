@@ -170,3 +170,25 @@ def test_create_reports_argument_errors_through_null_handle():
     assert L.biot_last_error(None) == b"order must be 2 or 4, got 3"
     with pytest.raises(BiotError, match="npml out of range"):
         Solver(64, 64, 64, 5.0, 4e-4, order=4, npml=40)
+
+
+def test_batch_create_reports_argument_errors_through_null_handle():
+    """biot_batch_create refuses bad configurations before any device call and leaves the
+    reason in biot_batch_last_error(NULL); the Python front end raises with it."""
+    from biot_amd._lib import BatchConfig
+    from biot_amd.batch import Batch
+
+    L = _lib.lib()
+    base = dict(nx=32, ny=32, nz=32, order=4, precision=1, npml=4, dx=5, dt=4e-4, pml_v0=4000,
+                pml_R=1e-6, Cv=1e8, Cp=1, device=0)
+    cases = [(dict(nx=7), "at least 8 cells"), (dict(order=3), "order must be 2 or 4"),
+             (dict(precision=3), "BIOT_FP32, BIOT_FP16 or BIOT_FP64"),
+             (dict(precision=4), "BIOT_FP32, BIOT_FP16 or BIOT_FP64"),
+             (dict(dt=0.0), "dx and dt must be positive"), (dict(npml=-1), "npml >= 0")]
+    for change, why in cases:
+        c = BatchConfig(**{**base, **change})
+        h = C.c_void_p()
+        assert L.biot_batch_create(C.byref(c), C.byref(h)) == 1 and not h.value
+        assert why in L.biot_batch_last_error(None).decode(), change
+    with pytest.raises(BiotError, match="order must be 2 or 4"):
+        Batch(32, 32, 32, 5.0, 4e-4, order=3)
