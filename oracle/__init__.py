@@ -11,7 +11,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+DEFAULT_LIB_PATH = os.path.join(HERE, "liboracle.so")
+# BIOT_ORACLE_LIB names another build of the same sources (the mutant checks of
+# tests/test_oracle_mutants.py load deliberately broken builds through it)
+LIB_PATH = os.environ.get("BIOT_ORACLE_LIB") or DEFAULT_LIB_PATH
 MODE = {"f32": 0, "f16": 1, "f64": 2}
 FIELDS = ["vx", "vy", "vz", "qx", "qy", "qz", "txx", "tyy", "tzz", "txy", "txz", "tyz", "p"]
 NCOEF = 23
@@ -21,7 +24,7 @@ DP = C.POINTER(C.c_double)
 
 def build():
     subprocess.run(["make", "-C", HERE, "-s"], check=True)
-    return LIB_PATH
+    return DEFAULT_LIB_PATH
 
 
 REF_SRC = "/root/reference"
@@ -139,6 +142,8 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
+            if LIB_PATH != DEFAULT_LIB_PATH:
+                raise FileNotFoundError(f"oracle library {LIB_PATH} (BIOT_ORACLE_LIB) is missing")
             build()
         L = C.CDLL(LIB_PATH)
         L.or_f32_to_f16.argtypes = [C.c_float]

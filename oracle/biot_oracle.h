@@ -28,9 +28,17 @@
  * Space order 2 is the reference's stencil; order 4 (coefficients 9/8, -1/24) has no
  * reference arithmetic and is pinned by closed-form tests only.
  *
- * Pins: every part above is checked by closed forms, invariants or brute force
- * (tests/test_oracle_pins.py); the Biot moduli are in addition checked bit for bit against
- * the reference's own outputfunc.cpp compiled from its sources (`make ref`, oracle/_ref).
+ * Pins (tests/test_oracle_pins.py): moduli by Gassmann's equation and hand values, and bit for
+ * bit against the reference's own outputfunc.cpp compiled from its sources (`make ref`,
+ * oracle/_ref); averages by scipy's harmonic mean; PML profiles by their design reflection;
+ * Ricker by its closed form; the fluid coupling (inertial term m - rho_f^2/rho, damping
+ * eta/kappa, rho_f/rho) by the exact solution of the Biot relaxation ODE, momentum
+ * conservation and the transient toward Darcy's flux along each axis; every difference of
+ * both half steps by plane-wave eigenmodes along every axis and polarisation, and by the
+ * C-PML closed form per derivative; the stress update by the constitutive law on linear
+ * fields.  tests/test_oracle_mutants.py plants 22 plausible mistakes (signs, inverted
+ * ratios, wrong strides or directions, wrong profile nodes) in copies of these sources and
+ * checks that the pins reject every one.
  * Parity unpinned: the rounding of the original nvcc-built binary (FMA contraction choices,
  * cuda_fp16's double-to-half stores), which cannot be built or run here.
  *
