@@ -244,23 +244,6 @@ def batch_shots(nt=1000, every=50, graph_steps=50):
                 checkpoint_every=every, gradient_groups=groups, misfit=J, finite=finite)
 
 
-def kernel_probe(steps):
-    """Each stencil variant at a fixed tile for `steps` steps of the 256^3 workload (for
-    hardware-counter passes): FP32 and FP16 one cell per lane (128x1 threads, 4 planes),
-    FP16 four cells per lane (64x4 threads, 2 planes)."""
-    from biot_amd import configs
-
-    cfg = configs.layered256(order=4, nt=steps)
-    out = {}
-    for prec, shape in (("fp32", (128, 1, 4, 1)), ("fp16", (128, 1, 4, 1)), ("fp16", (64, 4, 2, 4))):
-        s = configs.make_solver(cfg, precision=prec, nt=steps)
-        s.set_launch_shape(*shape)
-        v, t = s.profile_steps(steps)
-        out[f"{prec} xv{shape[3]}"] = {"velocity_ms": round(v, 4), "stress_ms": round(t, 4)}
-        s.close()
-    print(json.dumps(out))
-
-
 def stream_copy_rate(mib=1024):
     """HBM rate (GB/s, bytes read + written) of torch's device copy of `mib` MiB: what a plain
     streaming kernel reaches on this GPU.  Measured in a child process (`--copy-calib`) before
@@ -274,28 +257,6 @@ def stream_copy_rate(mib=1024):
         log(f"[bench] copy reference failed: {r.stderr.strip()[-300:]}")
         return None
     return json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"]
-
-
-def hip_copy_rates(mib=1024):
-    """HBM rate (GB/s, bytes read + written) of the library's streaming kernels, 20 timed
-    launches each: a 16-byte copy of `mib` MiB with plain and with nontemporal accesses, and the
-    velocity kernel's access pattern without its stencil (the 256^3 cells: 13 fields and the
-    packed coefficient block read, 6 fields written), FP32 and FP16, with plain and
-    nontemporal stores."""
-    import ctypes as C
-
-    from biot_amd import _lib
-
-    out = {}
-    for name, variant, nbytes in (("plain", 0, mib * 2 ** 20), ("nontemporal", 1, mib * 2 ** 20),
-                                  ("velocity_pattern", 2, 4 * 256 ** 3),
-                                  ("velocity_pattern_nt", 3, 4 * 256 ** 3),
-                                  ("velocity_pattern_fp16", 4, 4 * 256 ** 3),
-                                  ("velocity_pattern_fp16_nt", 5, 4 * 256 ** 3)):
-        g = C.c_double()
-        rc = _lib.lib().biot_copy_bandwidth(0, nbytes, 20, variant, C.byref(g))
-        out[name] = round(g.value, 1) if rc == 0 else None
-    return out
 
 
 def copy_calibration(mib):
@@ -326,7 +287,6 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB")
-    ap.add_argument("--kernel-probe", type=int, default=0, metavar="STEPS")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
@@ -341,9 +301,6 @@ def main():
     if args.copy_calib:
         copy_calibration(args.copy_calib)
         return
-    if args.kernel_probe:
-        kernel_probe(args.kernel_probe)
-        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solver holds the whole grid on a single MI355X "
               f"and has no multi-GPU path; got --gpus {args.gpus}", file=sys.stderr)
@@ -352,11 +309,7 @@ def main():
     from biot_amd import configs
 
     copy_gbs = stream_copy_rate()  # before this process touches the GPU
-    hip_copy = hip_copy_rates()
-    log(f"[bench] plain copies: torch {copy_gbs} GB/s, HIP 16-byte {hip_copy['plain']} GB/s, "
-        f"nontemporal {hip_copy['nontemporal']} GB/s; velocity access pattern without stencil "
-        f"{hip_copy['velocity_pattern']} GB/s, nontemporal stores {hip_copy['velocity_pattern_nt']} GB/s; "
-        f"FP16 pattern {hip_copy['velocity_pattern_fp16']} / {hip_copy['velocity_pattern_fp16_nt']} GB/s")
+    log(f"[bench] plain copy (torch): {copy_gbs} GB/s")
     cfg = configs.layered256(order=4, nt=args.warmup + args.steps, n=args.size)
     n = cfg["nx"] * cfg["ny"] * cfg["nz"]
     r32 = run_precision(cfg, "fp32", args.steps, args.warmup)
@@ -396,7 +349,6 @@ def main():
                           "frac": round(ach_t / HBM_PEAK_GBS, 4), "algorithmic_bytes": bt,
                           "mean_launch_ms": round(r32["kernel_ms_t"], 5)},
         "stream_copy_GBs": copy_gbs,
-        "hip_copy_GBs": hip_copy,
         "step_roofline": {"achieved_GBs": round(ach_step, 1), "frac": round(ach_step / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes_per_step": bstep,
                           "bytes_per_cell_update": round(bstep / upd, 2)},

@@ -31,14 +31,15 @@
 extern "C" {
 #endif
 
-#define BIOT_ABI_VERSION 1
+#define BIOT_ABI_VERSION 2
 
 enum biot_status {
   BIOT_OK = 0,
   BIOT_ERR_ARG = 1,        /* invalid argument (dims, order, indices, sizes) */
   BIOT_ERR_HIP = 2,        /* HIP runtime failure */
   BIOT_ERR_STATE = 3,      /* call out of order (e.g. run before set_medium) */
-  BIOT_ERR_NONFINITE = 4,  /* NaN/Inf detected on the receivers */
+  BIOT_ERR_NONFINITE = 4,  /* NaN/Inf detected on the receivers: the steps ran, and the
+                              message names the first step that recorded one */
   BIOT_ERR_RANGE = 5,      /* value outside the storage format's range (FP16 scaling) */
   BIOT_ERR_NOMEM = 6       /* device or host allocation failed */
 };
@@ -158,7 +159,10 @@ int biot_set_receivers(biot_handle *h, const int64_t *cells, int64_t nrec, uint3
 
 /* ---- time loop ---- */
 int biot_reset(biot_handle *h);             /* zero wavefields and PML memory, step = 0 */
-int biot_run(biot_handle *h, int32_t nsteps); /* advance nsteps: source -> V -> T -> record */
+/* Advance nsteps: source -> V -> T -> record.  Returns BIOT_ERR_NONFINITE when a receiver has
+ * recorded NaN or Inf since the last biot_reset / biot_set_receivers (the watchdog that
+ * replaces the reference's exit on a failed check, kernel.cu:42-52). */
+int biot_run(biot_handle *h, int32_t nsteps);
 int biot_step_count(biot_handle *h, int32_t *it);
 int biot_synchronize(biot_handle *h);
 /* Wall time (ms, HIP events on the handle's stream) of the last biot_run. */
@@ -290,11 +294,9 @@ int biot_batch_release(biot_batch *b);
 
 /* ---- launch shapes ---- */
 /* Tile shape of the two stencil kernels: bx*by threads per block (bx*by <= 256, bx a
- * multiple of 64), each thread marching zchunk planes and updating xv consecutive x cells
- * (1; or 2 and 4 for FP32/FP16, 4 for FP16C, when nx is a multiple of 4).  xv may be NULL
- * in biot_get_launch_shape. */
-int biot_set_launch_shape(biot_handle *h, int32_t bx, int32_t by, int32_t zchunk, int32_t xv);
-int biot_get_launch_shape(biot_handle *h, int32_t *bx, int32_t *by, int32_t *zchunk, int32_t *xv);
+ * multiple of 64), each thread marching zchunk planes of its column. */
+int biot_set_launch_shape(biot_handle *h, int32_t bx, int32_t by, int32_t zchunk);
+int biot_get_launch_shape(biot_handle *h, int32_t *bx, int32_t *by, int32_t *zchunk);
 /* Times every candidate tile shape for trial_steps steps on this grid and keeps the
  * fastest; results are cached per (precision, order, grid, GPU arch) in the file named by
  * $BIOT_TUNE_CACHE (default $HOME/.cache/biot_tune.txt), and a cached entry is reused
@@ -318,17 +320,6 @@ int biot_snapshot_xz(biot_handle *h, int32_t field, int32_t j, float *out);
 int biot_get_field_scales(biot_handle *h, float *scales13);
 /* Device bytes held by the handle. */
 int biot_device_bytes(biot_handle *h, int64_t *bytes);
-
-/* Measurement aid (no counterpart in the reference): the HBM rate, in GB/s of bytes read plus
- * bytes written, of a streaming kernel timed over `reps` launches after 3 warm-up launches on
- * device `device` (made current).  Variants 0 and 1: a device copy of `bytes` bytes (a
- * multiple of 16, at most 16 GiB), one 16-byte load and store per thread, ordinary (0) or
- * nontemporal (1) accesses.  Variants 2 and 3: the FP32 velocity kernel's access pattern
- * without its stencil, over bytes/4 cells (bytes at most 1 GiB): 13 float fields and a
- * 56-byte packed block read, 6 float fields written per cell (132 B), ordinary (2) or
- * nontemporal (3) stores.  Variants 4 and 5: the FP16 velocity kernel's pattern over bytes/4
- * cells: 13 binary16 fields and a 28-byte block read, 6 binary16 fields written (66 B). */
-int biot_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double *GBs);
 
 #ifdef __cplusplus
 }

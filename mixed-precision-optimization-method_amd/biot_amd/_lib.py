@@ -121,9 +121,8 @@ def lib():
         "biot_synchronize": ([H], C.c_int),
         "biot_last_run_ms": ([H, _DP], C.c_int),
         "biot_profile_steps": ([H, i32, _DP, _DP], C.c_int),
-        "biot_set_launch_shape": ([H, i32, i32, i32, i32], C.c_int),
-        "biot_get_launch_shape": ([H, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
-                                   C.POINTER(i32)], C.c_int),
+        "biot_set_launch_shape": ([H, i32, i32, i32], C.c_int),
+        "biot_get_launch_shape": ([H, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "biot_autotune": ([H, i32], C.c_int),
         "biot_read_traces": ([H, FP, i64], C.c_int),
         "biot_read_traces_f64": ([H, _DP, i64], C.c_int),
@@ -133,7 +132,6 @@ def lib():
         "biot_snapshot_xz": ([H, i32, i32, FP], C.c_int),
         "biot_get_field_scales": ([H, FP], C.c_int),
         "biot_device_bytes": ([H, C.POINTER(i64)], C.c_int),
-        "biot_copy_bandwidth": ([i32, i64, i32, i32, C.POINTER(C.c_double)], C.c_int),
         "biot_set_params": ([H, C.POINTER(Params)], C.c_int),
         "biot_set_medium_device": ([H, C.POINTER(Medium)], C.c_int),
         "biot_kernel_coefficients": ([C.c_float, i64, C.POINTER(FP), C.POINTER(_DP)], C.c_int),
@@ -186,7 +184,7 @@ def exported_symbols():
             "biot_set_launch_shape", "biot_get_launch_shape", "biot_autotune",
             "biot_read_traces", "biot_read_traces_f64", "biot_read_field",
             "biot_read_field_f64", "biot_write_field_f64", "biot_snapshot_xz",
-            "biot_get_field_scales", "biot_device_bytes", "biot_copy_bandwidth", "biot_write_text_2d",
+            "biot_get_field_scales", "biot_device_bytes", "biot_write_text_2d",
             "biot_write_text_xz", "biot_write_binary", "biot_run_stream_traces", "biot_run_stream",
             "biot_set_params",
             "biot_set_medium_device", "biot_kernel_coefficients", "biot_read_kernel_coefficients",

@@ -100,6 +100,12 @@ class Batch:
         obs = [np.ascontiguousarray(np.asarray(o, np.float32)) for o in observed]
         if len(obs) != len(self.shots):
             raise ValueError("one observed array per shot")
+        for i, (o, d) in enumerate(zip(obs, self.shots)):
+            # the library reads the shot's whole trace layout, ncomp x (the steps its receivers
+            # hold) x nrec samples, from each array
+            want = (d["ncomp"], d["nt"], d["nrec"])
+            if o.shape != want:
+                raise ValueError(f"shot {i}: observed traces have shape {o.shape}, expected {want}")
         optr = (C.POINTER(C.c_float) * len(obs))(*[fptr(o) for o in obs])
         grads = [np.zeros(self.n_model, np.float64) for _ in range(6)]
         gptr = (C.POINTER(C.c_double) * 6)(*[dptr(g) for g in grads])

@@ -145,12 +145,12 @@ class Solver:
         self._check(lib().biot_last_run_ms(self._h, C.byref(ms)), "biot_last_run_ms")
         return ms.value
 
-    def set_launch_shape(self, bx, by, zchunk, xv=1):
-        self._check(lib().biot_set_launch_shape(self._h, bx, by, zchunk, xv), "biot_set_launch_shape")
+    def set_launch_shape(self, bx, by, zchunk):
+        self._check(lib().biot_set_launch_shape(self._h, bx, by, zchunk), "biot_set_launch_shape")
 
     def launch_shape(self):
-        """(bx, by, zchunk, xv) of the stencil kernels."""
-        v = [C.c_int32() for _ in range(4)]
+        """(bx, by, zchunk) of the stencil kernels."""
+        v = [C.c_int32() for _ in range(3)]
         self._check(lib().biot_get_launch_shape(self._h, *[C.byref(x) for x in v]),
                     "biot_get_launch_shape")
         return tuple(x.value for x in v)

@@ -497,6 +497,7 @@ extern "C" int biot_set_receivers(biot_handle *h, const int64_t *cells, int64_t 
       HIPCHK(h, hipMalloc((void **)&h->d_traces64, tr * sizeof(double)));
       HIPCHK(h, hipMemsetAsync(h->d_traces64, 0, tr * sizeof(double), h->stream));
     }
+    if (int rc = h->clear_nonfinite()) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
   }
   return BIOT_OK;
@@ -519,6 +520,7 @@ extern "C" int biot_reset(biot_handle *h)
   if (h->d_traces64)
     HIPCHK(h, hipMemsetAsync(h->d_traces64, 0,
                              (size_t)h->ncomp * h->nt_max * h->nrec * sizeof(double), h->stream));
+  if (int rc = h->clear_nonfinite()) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->it = 0;
   return BIOT_OK;
@@ -527,29 +529,9 @@ extern "C" int biot_reset(biot_handle *h)
 int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks)
 {
   SourceArgs sa;
-  std::memset(&sa, 0, sizeof sa);
+  source_args(sa);
   RecordArgs ra;
-  std::memset(&ra, 0, sizeof ra);
-  for (int f = 0; f < NFIELD; ++f) {
-    sa.f[f] = field_ptr(f);
-    ra.f[f] = field_ptr(f);
-    sa.lo[f] = lo_ptr(f);
-    ra.lo[f] = lo_ptr(f);
-    sa.inv_scale[f] = 1.0f / fscale[f];
-  }
-  sa.cell = src_cell;
-  sa.mask = src_mask;
-  sa.wavelet = (const float *)d_wavelet;
-  ra.rec = (const int64_t *)d_rec;
-  ra.nrec = nrec;
-  ra.ncomp = ncomp;
-  for (int c = 0; c < ncomp; ++c) {
-    ra.comp[c] = comp[c];
-    ra.scale[c] = fscale[comp[c]];
-  }
-  ra.traces = (float *)d_traces;
-  ra.traces64 = d_traces64;
-  ra.nt_max = nt_max;
+  record_args(ra);
 
   StepArgs args;
   fill_args(args);
@@ -594,7 +576,7 @@ extern "C" int biot_run(biot_handle *h, int32_t nsteps)
   float ms = 0;
   HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
-  return BIOT_OK;
+  return h->check_nonfinite();
 }
 
 extern "C" int biot_profile_steps(biot_handle *h, int32_t nsteps, double *ms_v, double *ms_t)
@@ -636,31 +618,21 @@ static bool shape_ok(int bx, int by, int zc)
 {
   return bx >= 64 && bx % 64 == 0 && by >= 1 && bx * by <= 256 && zc >= 1 && zc <= 4096;
 }
-static bool xv_ok(const biot_handle *h, int xv)
-{
-  return xv == 1 || xv_supported(h->prec, xv, h->g.nx);
-}
-
-extern "C" int biot_set_launch_shape(biot_handle *h, int32_t bx, int32_t by, int32_t zchunk,
-                                     int32_t xv)
+extern "C" int biot_set_launch_shape(biot_handle *h, int32_t bx, int32_t by, int32_t zchunk)
 {
   if (!h) return BIOT_ERR_ARG;
   if (!shape_ok(bx, by, zchunk))
     return fail(h, BIOT_ERR_ARG, "launch shape %dx%d, zchunk %d not allowed", bx, by, zchunk);
-  if (!xv_ok(h, xv))
-    return fail(h, BIOT_ERR_ARG, "%d cells per lane not available for this precision and nx", xv);
-  h->shape = LaunchShape{bx, by, zchunk, xv};
+  h->shape = LaunchShape{bx, by, zchunk};
   return BIOT_OK;
 }
 
-extern "C" int biot_get_launch_shape(biot_handle *h, int32_t *bx, int32_t *by, int32_t *zchunk,
-                                     int32_t *xv)
+extern "C" int biot_get_launch_shape(biot_handle *h, int32_t *bx, int32_t *by, int32_t *zchunk)
 {
   if (!h || !bx || !by || !zchunk) return BIOT_ERR_ARG;
   *bx = h->shape.bx;
   *by = h->shape.by;
   *zchunk = h->shape.zchunk;
-  if (xv) *xv = h->shape.xv;
   return BIOT_OK;
 }
 
@@ -679,11 +651,11 @@ std::string tune_key(const biot_handle *h)
   for (auto &ch : arch)
     if (ch == ' ') ch = '_';
   char buf[256];
-  snprintf(buf, sizeof buf, "v2 %s p%d o%d %dx%dx%d", arch.c_str(), h->prec, h->g.order, h->g.nx,
+  snprintf(buf, sizeof buf, "v3 %s p%d o%d %dx%dx%d", arch.c_str(), h->prec, h->g.order, h->g.nx,
            h->g.ny, h->g.nz);
   return buf;
 }
-bool tune_lookup(const biot_handle *h, const std::string &key, LaunchShape &out)
+bool tune_lookup(const std::string &key, LaunchShape &out)
 {
   FILE *f = std::fopen(tune_cache_path().c_str(), "r");
   if (!f) return false;
@@ -693,10 +665,9 @@ bool tune_lookup(const biot_handle *h, const std::string &key, LaunchShape &out)
     std::string s(line);
     if (s.compare(0, key.size(), key) != 0 || s.size() <= key.size() || s[key.size()] != ' ')
       continue;
-    int bx, by, zc, xv;
-    if (std::sscanf(s.c_str() + key.size(), " %d %d %d %d", &bx, &by, &zc, &xv) == 4 &&
-        shape_ok(bx, by, zc) && xv_ok(h, xv)) {
-      out = LaunchShape{bx, by, zc, xv};
+    int bx, by, zc;
+    if (std::sscanf(s.c_str() + key.size(), " %d %d %d", &bx, &by, &zc) == 3 && shape_ok(bx, by, zc)) {
+      out = LaunchShape{bx, by, zc};
       found = true;  // the last entry for a key wins
     }
   }
@@ -714,7 +685,7 @@ void tune_store(const std::string &key, LaunchShape s, double ms)
       if (k == dir.size() || dir[k] == '/') (void)mkdir(dir.substr(0, k).c_str(), 0755);
   }
   if (FILE *f = std::fopen(path.c_str(), "a")) {
-    std::fprintf(f, "%s %d %d %d %d %.6f\n", key.c_str(), s.bx, s.by, s.zchunk, s.xv, ms);
+    std::fprintf(f, "%s %d %d %d %.6f\n", key.c_str(), s.bx, s.by, s.zchunk, ms);
     std::fclose(f);
   }
 }
@@ -727,13 +698,12 @@ extern "C" int biot_autotune(biot_handle *h, int32_t trial_steps)
   if (!h->have_coef) return fail(h, BIOT_ERR_STATE, "biot_autotune before set_medium");
   const std::string key = tune_key(h);
   LaunchShape best{};
-  if (tune_lookup(h, key, best)) {
+  if (tune_lookup(key, best)) {
     h->shape = best;
     return biot_reset(h);
   }
   static const int shapes[][2] = {{64, 4}, {64, 2}, {64, 1}, {128, 2}, {128, 1}, {256, 1}};
   static const int chunks[] = {1, 2, 4, 8, 16, 32, 64};
-  static const int xvs[] = {1, 2, 4};
   const int nzi = h->g.nz - h->g.order;
   double best_ms = 1e30;
   const LaunchShape keep = h->shape;
@@ -741,12 +711,10 @@ extern "C" int biot_autotune(biot_handle *h, int32_t trial_steps)
   HIPCHK(h, hipEventCreate(&e0));
   HIPCHK(h, hipEventCreate(&e1));
   int rc = BIOT_OK;
-  for (int xv : xvs) {
-   if (!xv_ok(h, xv)) continue;
-   for (auto &sh : shapes) {
+  for (auto &sh : shapes) {
     for (int zc : chunks) {
       if (zc > 2 * nzi && zc != chunks[0]) continue;
-      h->shape = LaunchShape{sh[0], sh[1], zc, xv};
+      h->shape = LaunchShape{sh[0], sh[1], zc};
       rc = h->enqueue_steps(1, nullptr);  // warm-up
       if (!rc && hipEventRecord(e0, h->stream) != hipSuccess) rc = BIOT_ERR_HIP;
       if (!rc) rc = h->enqueue_steps(trial_steps, nullptr);
@@ -761,8 +729,6 @@ extern "C" int biot_autotune(biot_handle *h, int32_t trial_steps)
       }
     }
     if (rc) break;
-   }
-   if (rc) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -417,7 +417,7 @@ extern "C" int biot_batch_set_launch(biot_batch *b, int32_t bx, int32_t by, int3
   if (!b) return BIOT_ERR_ARG;
   if (bx < 64 || bx % 64 || by < 1 || bx * by > 256 || zchunk < 1 || blocks < 8 || blocks % 8)
     return bfail(b, BIOT_ERR_ARG, "launch %dx%d, zchunk %d, %d blocks not allowed", bx, by, zchunk, blocks);
-  b->shape = LaunchShape{bx, by, zchunk, 1};
+  b->shape = LaunchShape{bx, by, zchunk};
   b->blocks = blocks;
   b->built = false;
   b->drop_graph();
@@ -452,7 +452,7 @@ extern "C" int biot_batch_autotune(biot_batch *b, int32_t trial_steps)
     for (int zc : chunks)
       for (int gblocks : grids) {
         if (rc) break;
-        b->shape = LaunchShape{sh[0], sh[1], zc, 1};
+        b->shape = LaunchShape{sh[0], sh[1], zc};
         b->built = false;
         b->drop_graph();
         if ((rc = build_tables(b))) break;
@@ -498,6 +498,12 @@ extern "C" int biot_batch_forward(biot_batch *b, int32_t nt, int32_t graph_steps
   BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_ms = ms;
   for (auto &s : b->shots) s.h->it = nt;
+  for (size_t i = 0; i < b->shots.size(); ++i)
+    if ((rc = b->shots[i].h->check_nonfinite())) {
+      char what[32];
+      snprintf(what, sizeof what, "shot %d", (int)i);
+      return hfail(b, b->shots[i].h, rc, what);
+    }
   return BIOT_OK;
 }
 

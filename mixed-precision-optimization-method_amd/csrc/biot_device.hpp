@@ -80,20 +80,12 @@ inline size_t carry_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PRE
 // Host-side launchers (biot_kernels.hip).
 struct LaunchShape {
   int bx, by, zchunk;
-  int xv = 1;  // cells per lane along x (1: scalar kernels; 2, 4: x-vectorised kernels)
 };
 
 hipError_t launch_velocity(int prec, int order, const StepArgs &args, LaunchShape shape,
                            hipStream_t s);
 hipError_t launch_stress(int prec, int order, const StepArgs &args, LaunchShape shape,
                          hipStream_t s);
-// x-vectorised kernels (biot_kernels_xv.hip): FP32 and FP16 with xv 2 or 4, FP16C with 4;
-// nx must be a multiple of 4
-bool xv_supported(int prec, int xv, int nx);
-hipError_t launch_velocity_xv(int prec, int order, const StepArgs &args, LaunchShape shape,
-                              hipStream_t s);
-hipError_t launch_stress_xv(int prec, int order, const StepArgs &args, LaunchShape shape,
-                            hipStream_t s);
 struct SourceArgs {
   void *f[NFIELD];
   int8_t *lo[NFIELD];
@@ -114,8 +106,12 @@ struct RecordArgs {
   float scale[NFIELD];      // storage scale per recorded component
   float *traces;            // device [comp][nt_max][nrec]
   double *traces64;         // same layout in double (FP64 handles), or null
+  int *nonfinite;           // device: the first step with a non-finite sample (NONFINITE_NONE:
+                            // none so far), or null
   int it, nt_max;
 };
+// the byte pattern a cleared non-finite flag holds (every byte 0x7F): larger than any step
+constexpr int NONFINITE_NONE = 0x7F7F7F7F;
 hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s);
 
 // GPU-side set-up (biot_build.hip), bit-identical to the host builder.

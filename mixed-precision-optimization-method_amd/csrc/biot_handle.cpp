@@ -70,7 +70,33 @@ void biot_handle::record_args(RecordArgs &ra) const
   }
   ra.traces = (float *)d_traces;
   ra.traces64 = d_traces64;
+  ra.nonfinite = d_flag;
   ra.nt_max = nt_max;
+}
+
+int biot_handle::clear_nonfinite()
+{
+  if (hipMemsetAsync(d_flag, 0x7F, sizeof(int), stream) != hipSuccess) {
+    err = "clearing the non-finite flag failed";
+    return BIOT_ERR_HIP;
+  }
+  return BIOT_OK;
+}
+
+int biot_handle::check_nonfinite()
+{
+  int first = NONFINITE_NONE;
+  if (hipStreamSynchronize(stream) != hipSuccess ||
+      hipMemcpy(&first, d_flag, sizeof first, hipMemcpyDeviceToHost) != hipSuccess) {
+    err = "reading the non-finite flag failed";
+    return BIOT_ERR_HIP;
+  }
+  if (first == NONFINITE_NONE) return BIOT_OK;
+  char buf[160];
+  snprintf(buf, sizeof buf, "non-finite value on the receivers at step %d (the steps ran; traces and fields hold it)",
+           first);
+  err = buf;
+  return BIOT_ERR_NONFINITE;
 }
 
 size_t biot_handle::lam_bytes() const
@@ -197,6 +223,7 @@ biot_handle::~biot_handle()
   if (d_lo) (void)hipFree(d_lo);
   if (d_coef) (void)hipFree(d_coef);
   if (d_kpack) (void)hipFree(d_kpack);
+  if (d_flag) (void)hipFree(d_flag);
   for (auto *p : d_carry)
     if (p) (void)hipFree(p);
   for (auto &ax : d_prof)
@@ -265,7 +292,8 @@ int biot_handle::allocate()
     return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
   };
   bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, (size_t)NKC * n * esz) &&
-            alloc(&d_kpack, (size_t)NKC * n * esz);
+            alloc(&d_kpack, (size_t)NKC * n * esz) && alloc((void **)&d_flag, 256) &&
+            clear_nonfinite() == BIOT_OK;
   if (ok && comp_fields(prec)) ok = alloc((void **)&d_lo, (size_t)NFIELD * n);
   for (int ax = 0; ax < 3 && ok; ++ax) ok = alloc(&d_carry[ax], (size_t)NCARRY * slab[ax] * csz);
   const int dims[3] = {g.nx, g.ny, g.nz};

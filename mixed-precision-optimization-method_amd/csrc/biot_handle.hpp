@@ -26,6 +26,8 @@ struct biot_handle {
   void *d_coef = nullptr;            // 22 * n
   void *d_kpack = nullptr;           // the coefficients packed per cell (velocity, then stress block)
   void *d_carry[3] = {nullptr, nullptr, nullptr};
+  int *d_flag = nullptr;             // first step with a non-finite receiver sample
+                                     // (biot::NONFINITE_NONE: none since the last clear)
   void *d_prof[3][4] = {};           // e_int, a_int, e_half, a_half per axis
   int64_t slab[3] = {0, 0, 0};
   int L[3] = {0, 0, 0}, R[3] = {0, 0, 0};
@@ -94,6 +96,8 @@ struct biot_handle {
   // argument blocks shared by the single-grid and batched drivers
   void source_args(biot::SourceArgs &sa) const;
   void record_args(biot::RecordArgs &ra) const;
+  int clear_nonfinite();                     // enqueue a clear of d_flag on the stream
+  int check_nonfinite();                     // synchronize; BIOT_ERR_NONFINITE if it was set
   size_t lam_bytes() const;                  // adjoint wavefields, carries, derivative adjoints
   int alloc_adjoint(int every);              // d_slots (every+1 states), d_lam, d_grad, d_adjsrc
   void adj_args(biot::AdjArgs &A) const;     // after alloc_adjoint

@@ -653,8 +653,12 @@ __device__ __forceinline__ void record_one(const RecordArgs &ra, int64_t r, int 
   const double v = (double)load_raw<P>(ra.f[f], ra.lo[f], (P::cmask >> f) & 1u, (u32)ra.rec[r]) *
                    (double)ra.scale[ci];
   const int64_t o = ((int64_t)ci * ra.nt_max + it) * ra.nrec + r;
-  ra.traces[o] = (float)v;
+  const float vf = (float)v;
+  ra.traces[o] = vf;
   if (ra.traces64) ra.traces64[o] = v;
+  // the watchdog of the reference's abort on a bad state (kernel.cu:42-52): the first step
+  // at which a receiver sees NaN or Inf
+  if (ra.nonfinite && !__builtin_isfinite(vf)) atomicMin(ra.nonfinite, it);
 }
 template <class P>
 __global__ void k_record(const RecordArgs ra)
@@ -875,12 +879,10 @@ hipError_t launch_set_counter(int *it, int v, hipStream_t s)
 
 hipError_t launch_velocity(int prec, int order, const StepArgs &args, LaunchShape sh, hipStream_t s)
 {
-  if (sh.xv > 1) return launch_velocity_xv(prec, order, args, sh, s);
   return dispatch<true>(prec, order, args, sh, s);
 }
 hipError_t launch_stress(int prec, int order, const StepArgs &args, LaunchShape sh, hipStream_t s)
 {
-  if (sh.xv > 1) return launch_stress_xv(prec, order, args, sh, s);
   return dispatch<false>(prec, order, args, sh, s);
 }
 

@@ -214,6 +214,7 @@ extern "C" int biot_run_stream(biot_handle *h, int32_t nsteps, int32_t chunk, fl
     if (hipStreamSynchronize(side) != hipSuccess) rc = BIOT_ERR_HIP;
     else std::memcpy(snaps, h_snap, (size_t)nsnap * slice * sizeof(float));
   }
+  if (!rc) rc = h->check_nonfinite();
   cleanup();
   if (rc == BIOT_ERR_HIP && h->err.empty()) h->err = "streamed copy failed";
   return rc;

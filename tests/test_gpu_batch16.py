@@ -1,0 +1,147 @@
+"""GPU checks of the batched multi-shot workload at its own size.
+
+Workload (configs.batch16() as is): 16 shots in cubic windows of 96..192 cells (seeded) cut
+from the 512 x 512 x 384 perturbed medium (layers plus seeded +-10 % Gaussian-correlated
+perturbations of Kb, mu, phi), order 4, PML 20, 20 Hz explosive source at each window's
+centre, receivers every 8 cells on the window plane z = 20 recording vx, vz, p, 1000 steps,
+FP16 wavefield storage, FP32 arithmetic, FP64 gradient sums.
+
+* The graph-captured batched forward reproduces each of the 16 shots' single-grid runs.
+* The FP16 batched gradient against the FP32 batched gradient of the same data (observed:
+  the FP32 batched forward of the model with Kb raised 5 %).
+* The FP32 batched gradient of one window against fourth-order central differences of the
+  FP64 single-grid misfit at 4 seeded cells, for every parameter.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from biot_amd import configs, ricker
+from biot_amd._lib import PARAM_NAMES
+from biot_amd.solver import Solver
+from conftest import rel_l2
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+NT = 1000
+
+
+@pytest.fixture(scope="module")
+def cfg():
+    return configs.batch16(nt=NT)
+
+
+def _single(cfg, shot, precision, params=None):
+    wx, wy, wz = shot["size"]
+    s = Solver(wx, wy, wz, cfg["dx"], cfg["dt"], order=cfg["order"], precision=precision, npml=cfg["npml"])
+    s.set_params(configs.window_params(cfg, shot) if params is None else params)
+    si, sj, sk = shot["src"]
+    s.set_source(s.cell(si, sj, sk), ricker(cfg["f0"], cfg["dt"], cfg["nt"]), cfg["src_fields"])
+    rec = np.asarray(shot["rec"])
+    s.set_receivers(rec[:, 0] + rec[:, 1] * wx + rec[:, 2] * wx * wy, cfg["rec_fields"], cfg["nt"])
+    return s
+
+
+def _true(cfg):
+    m = dict(cfg["medium"])
+    m["Kb"] = m["Kb"] * 1.05
+    return dict(cfg, medium=m)
+
+
+def test_batch16_forward_equals_single_grids(cfg):
+    sides = [sh["size"][0] for sh in cfg["shots"]]
+    print(f"batch16 forward fp16: {len(sides)} shots, sides {min(sides)}..{max(sides)}, {NT} steps", flush=True)
+    b = configs.make_batch(cfg, precision="fp16")
+    b.forward(NT, graph_steps=50)
+    got = [b.traces(i) for i in range(len(cfg["shots"]))]
+    b.close()
+    worst = 0.0
+    for i, sh in enumerate(cfg["shots"]):
+        s = _single(cfg, sh, "fp16")
+        s.run(NT)
+        ref = s.traces()
+        s.close()
+        e = rel_l2(got[i], ref)
+        worst = max(worst, e)
+        print(f"  shot {i:2d} side {sh['size'][0]:3d}: rel L2 vs single grid {e:.2e}, "
+              f"max |diff| {float(np.abs(got[i] - ref).max()):.2e}", flush=True)
+        assert np.isfinite(got[i]).all() and np.abs(ref).max() > 0
+    print(f"  worst {worst:.2e}")
+    assert worst <= 1e-7
+
+
+def test_batch16_fp16_gradient_against_fp32(cfg):
+    """Measured on MI355X: 0.7-1.2 % per parameter (FP16 wavefields carry ~1e-3 relative
+    error in the recomputed forward fields the gradient sums read); bar 2.5e-2."""
+    print("batch16 gradient fp16 vs fp32", flush=True)
+    bt = configs.make_batch(_true(cfg), precision="fp32")
+    bt.forward(NT, graph_steps=50)
+    obs = [bt.traces(i) for i in range(len(cfg["shots"]))]
+    bt.close()
+    grads = {}
+    for prec in ("fp32", "fp16"):
+        b = configs.make_batch(cfg, precision=prec)
+        grads[prec] = b.gradient(NT, obs, every=50)
+        print(f"  {prec}: J {grads[prec][0]:.6e}, {b.last_ms():.0f} ms, {b.gradient_groups()} group(s)", flush=True)
+        b.close()
+    (J32, g32), (J16, g16) = grads["fp32"], grads["fp16"]
+    errs = {k: rel_l2(g16[k], g32[k]) for k in g32}
+    print("  fp16 vs fp32 gradient rel L2: " + " ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    assert all(np.isfinite(g).all() for g in g16.values())
+    assert abs(J16 - J32) <= 0.02 * J32
+    assert max(errs.values()) <= 2.5e-2
+
+
+def test_batch16_fp32_gradient_matches_finite_differences(cfg):
+    """The smallest window (96 cells) alone as a one-shot FP32 batch: its gradient at 4
+    seeded cells inside the window (outside the absorbing layers), every parameter, against
+    fourth-order central differences (step 1e-3 of the value) of the FP64 single-grid
+    misfit.  The residual is ~5 % of the data here, so the FP32 traces' ~3e-6 error does not
+    limit the comparison; bar 1e-2."""
+    which = int(np.argmin([sh["size"][0] for sh in cfg["shots"]]))
+    shot = cfg["shots"][which]
+    one = dict(cfg, shots=[shot])
+    print(f"batch16 shot {which} (side {shot['size'][0]}) fp32 gradient vs fd", flush=True)
+    t = _single(_true(cfg), shot, "fp32")
+    t.run(NT)
+    obs = t.traces()
+    t.close()
+    b = configs.make_batch(one, precision="fp32")
+    J, g = b.gradient(NT, [obs], every=50)
+    b.close()
+    x0, y0, z0 = shot["origin"]
+    p0 = configs.window_params(cfg, shot)
+    s64 = _single(cfg, shot, "fp64")
+    obs64 = obs.astype(np.float64)
+
+    def fwd(p):
+        s64.set_params(p)
+        s64.reset()
+        s64.run(NT)
+        return s64.traces_f64()
+
+    def dJ(da, db):
+        return math.fsum(((da - db) * (0.5 * (da + db) - obs64)).ravel())
+
+    side = shot["size"][0]
+    rng = np.random.default_rng(16)
+    lo, hi = cfg["npml"] + 4, side - cfg["npml"] - 4
+    worst = 0.0
+    for _ in range(4):
+        i, j, k = (int(v) for v in rng.integers(lo, hi, 3))
+        for name in PARAM_NAMES:
+            h = 1e-3 * abs(p0[name][k, j, i])
+            d = {}
+            for m in (1, -1, 2, -2):
+                p = {kk: v.copy() for kk, v in p0.items()}
+                p[name][k, j, i] += m * h
+                d[m] = fwd(p)
+            fd = (8 * dJ(d[1], d[-1]) - dJ(d[2], d[-2])) / (12 * h)
+            adj = g[name][z0 + k, y0 + j, x0 + i]
+            e = abs(adj - fd) / abs(fd)
+            worst = max(worst, e)
+            print(f"  ({i},{j},{k}) {name:6s} fd {fd:+.6e} batch {adj:+.6e} rel {e:.1e}", flush=True)
+    s64.close()
+    print(f"  worst {worst:.2e}")
+    assert worst <= 1e-2

@@ -87,3 +87,32 @@ def test_text_files_of_a_run(tmp_path):
     rows = [r for r in (tmp_path / "txx.dat").read_text().split("\n") if r]
     assert np.allclose(np.array(rows[32].split(), np.float32), snap[32], rtol=1e-6, atol=0)
     s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_nonfinite_watchdog(precision):
+    """A NaN sample in the wavelet (forced from the host; NaN arithmetic does not fault) reaches
+    the receiver at the source cell at that step: biot_run reports BIOT_ERR_NONFINITE naming the
+    step, so does the streamed loop, and biot_reset clears the flag."""
+    print(f"watchdog {precision}", flush=True)
+    cfg = configs.fwd64(order=2, nt=20)
+    s = configs.make_solver(cfg, precision=precision, nt=20)
+    cell = int(configs.cells(cfg, cfg["src"])[0])
+    w = np.ones(20, np.float32)
+    w[5] = np.nan
+    s.set_source(cell, w, ("txx",))
+    s.set_receivers([cell], ("txx",), 20)
+    with pytest.raises(BiotError) as e:
+        s.run(20)
+    assert e.value.status == 4 and "at step 5" in str(e.value)
+    s.reset()
+    s.run(4)  # steps 0..3 are clean again after the reset
+    with pytest.raises(BiotError) as e:
+        s.run_stream_traces(16, 5)
+    assert e.value.status == 4 and "at step 5" in str(e.value)
+    w[5] = 1.0
+    s.set_source(cell, w, ("txx",))
+    s.reset()
+    s.run(20)
+    assert np.isfinite(s.traces()).all()
+    s.close()
