@@ -36,12 +36,17 @@ def slab_counts(n, L, R, h):
 
 
 # element bytes per storage mode: field, coefficient, C-PML carry, residual (FP16C on the 10
-# compensated fields v, tau, p; FP16T on the 7 of tau, p); FP16 keeps every array in
-# binary16, FP16C and FP16T keep float carries
+# compensated fields v, tau, p; FP16T on the 7 of tau, p; FP16M's third byte of the 24-bit
+# tau, p); FP16 keeps every array in binary16, FP16C, FP16T and FP16M keep float carries
 WIDTHS = {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 2, 0), "fp16c": (2, 2, 4, 1), "fp16t": (2, 2, 4, 1),
-          "fp64": (8, 8, 8, 0)}
+          "fp16m": (2, 2, 4, 1), "fp64": (8, 8, 8, 0)}
 # residual accesses per updated cell (velocity kernel, stress kernel, one fused step)
-RESIDUAL_ACCESSES = {"fp16c": (13, 17, 20), "fp16t": (7, 14, 14)}
+RESIDUAL_ACCESSES = {"fp16c": (13, 17, 20), "fp16t": (7, 14, 14), "fp16m": (7, 14, 14)}
+# medium bytes per updated cell where they are not the coefficient arrays: FP16M reads the
+# cell's moduli records (velocity: rho, rho_f, C1, C2 in 8 bytes; stress: C, M, lambda_c, H in
+# 8 bytes and 1/mu in 2; the neighbours' records the averages use are re-reads of the same
+# arrays)
+MEDIUM_BYTES = {"fp16m": (8, 10, 18)}
 
 
 def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):
@@ -56,16 +61,46 @@ def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):
     FP16T per access of tau, p (V: 7 reads, T: 7 + 7, step 14)."""
     wf, wk, wc, wr = WIDTHS[mode]
     rv, rt, rstep = RESIDUAL_ACCESSES.get(mode, (0, 0, 0))
+    kv, kt, kstep = MEDIUM_BYTES.get(mode, (14 * wk, 8 * wk, 22 * wk))
     h = order // 2
     ni, nj, nk = nx - 2 * h, ny - 2 * h, nz - 2 * h
     upd = ni * nj * nk
     Lx, Rx, Ly, Ry, Lz, Rz = npml_cells
     memb = (slab_counts(nx, Lx, Rx, h) * nj * nk + slab_counts(ny, Ly, Ry, h) * ni * nk +
             slab_counts(nz, Lz, Rz, h) * ni * nj)
-    v = upd * (19 * wf + 14 * wk + rv * wr) + memb * 8 * wc
-    t = upd * (20 * wf + 8 * wk + rt * wr) + memb * 8 * wc
-    step = upd * (26 * wf + 22 * wk + rstep * wr) + memb * 16 * wc
+    v = upd * (19 * wf + kv + rv * wr) + memb * 8 * wc
+    t = upd * (20 * wf + kt + rt * wr) + memb * 8 * wc
+    step = upd * (26 * wf + kstep + rstep * wr) + memb * 16 * wc
     return v, t, step, upd, memb
+
+
+def adjoint_bytes(nx, ny, nz, order, npml_cells, nt, every):
+    """Compulsory HBM bytes of the adjoint sweep of an FP32 handle (nt steps, checkpoints every
+    `every`): per segment a state load (every field and C-PML carry read and written); per
+    step a state save, the recomputed forward step (the two stencil kernels) and the four
+    adjoint passes; per segment a flush of the float partial sums into the FP64 sums.
+
+    Adjoint passes per updated cell (4-byte adjoint fields, derivative adjoints G, coefficients
+    and partial sums; neighbours' values are re-reads): stress-local reads 7 lambda(tau, p),
+    8 coefficients, 6 forward v/q, 8 partial sums and writes 12 G and 8 sums (49 accesses);
+    stress-gather reads 6 lambda(v, q) and 12 G, writes 6 lambda (24); velocity-local reads
+    14 coefficients, 6 lambda(v, q), 6 forward q (two states), 7 forward tau/p, 14 partial sums
+    and writes 3 lambda(q), 12 G, 14 sums (76); velocity-gather reads 7 lambda(tau, p) and 12 G,
+    writes 7 lambda (26): 175 accesses = 700 B.  Per slab membership and adjoint half step: 4
+    forward carries read, 4 carry adjoints read and written (12 accesses, 24 per step = 96 B).
+    Flush per updated cell: 22 float partial sums read and cleared, 22 FP64 sums read and
+    written (22 x (4 + 4 + 8 + 8) = 528 B)."""
+    h = order // 2
+    v, t, _, upd, memb = algorithmic_bytes(nx, ny, nz, order, npml_cells, "fp32")
+    n = nx * ny * nz
+    Lx, Rx, Ly, Ry, Lz, Rz = npml_cells
+    slab_cells = (Lx + Rx) * ny * nz + nx * (Ly + Ry) * nz + nx * ny * (Lz + Rz)
+    state = 13 * n * 4 + 8 * slab_cells * 4
+    nseg = (nt + every - 1) // every
+    per_step = 2 * state + v + t + 700 * upd + 96 * memb
+    total = nseg * 2 * state + nt * per_step + nseg * 528 * upd
+    del h
+    return total, per_step, upd
 
 
 PROFILE = os.path.join(ROOT, "profiles", "stencil_kernels_layered256.md")
@@ -167,22 +202,29 @@ def cpu_baseline(cfg, nsteps):
                        f"(-O3, OpenMP), {dt:.1f} s", host=host_cpu())
 
 
-def adjoint_shot(repeats=2):
+def adjoint_shot(repeats=2, fd_cells=2):
     """Adjoint gradient of the 128^3, 1000-step sphere workload (checkpoints every 50, FP32
     wavefields, FP64 gradient sums): forward with checkpoints, FP64 misfit, segment
-    recompute and adjoint sweep, per shot.  Observed traces come from the true model."""
+    recompute and adjoint sweep, per shot.  Observed traces come from the FP64 run of the true
+    model.  The sweep's roofline uses adjoint_bytes; the FP32 gradient is checked against
+    central differences of the FP64 forward misfit (step 1e-3 of the value) at `fd_cells`
+    seeded cells for lambda and rho."""
+    import math
+
     import numpy as np
 
     from biot_amd import configs, media
 
     true, start = configs.adjoint128(), configs.adjoint128(sphere=False)
     nt, every = start["nt"], start["every"]
+    s64 = configs.make_solver(start, precision="fp64")
+    s64.set_params(media.params_from_medium(true["medium"]))
+    s64.reset()
+    s64.run(nt)
+    obs = s64.traces_f64().astype(np.float32)
+    p0 = media.params_from_medium(start["medium"])
     s = configs.make_solver(start, precision="fp32")
-    s.set_params(media.params_from_medium(true["medium"]))
-    s.reset()
-    s.run(nt)
-    obs = s.traces()
-    s.set_params(media.params_from_medium(start["medium"]))
+    s.set_params(p0)
     best = None
     for _ in range(repeats):
         t0 = time.perf_counter()
@@ -194,15 +236,51 @@ def adjoint_shot(repeats=2):
         wall = (time.perf_counter() - t0) * 1e3
         r = dict(ms_per_shot=round(f_ms + a_ms, 2), forward_checkpointed_ms=round(f_ms, 2),
                  adjoint_ms=round(a_ms, 2), wall_ms_per_shot=round(wall, 1), misfit=J,
-                 finite=bool(all(np.isfinite(v).all() for v in g.values())))
+                 finite=bool(all(np.isfinite(v).all() for v in g.values())), grad=g)
         if best is None or r["ms_per_shot"] < best["ms_per_shot"]:
             best = r
+    widths = slab_widths(s)
     s.close()
+    g = best.pop("grad")
+    total, per_step, upd = adjoint_bytes(start["nx"], start["ny"], start["nz"], start["order"], widths, nt, every)
+    ach = total / (best["adjoint_ms"] * 1e-3) / 1e9
+    best["roofline"] = {"bound": "hbm", "kernel": "adjoint sweep (state copies, recomputed forward steps, "
+                        "four adjoint passes, flushes)", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes": total, "bytes_per_step": per_step,
+                        "bytes_per_cell_step": round(per_step / upd, 1)}
+    # central differences of the FP64 misfit at seeded cells
+    obs64 = obs.astype(np.float64)
+    rng = np.random.default_rng(2024)
+    worst = 0.0
+    checks = []
+    for _ in range(fd_cells):
+        i, j, k = (int(v) for v in rng.integers(24, start["nx"] - 24, 3))
+        for name in ("lambda", "rho"):
+            hstep = 1e-3 * abs(p0[name][k, j, i])
+            d = {}
+            for m in (1, -1):
+                p = {kk: v.copy() for kk, v in p0.items()}
+                p[name][k, j, i] += m * hstep
+                s64.set_params(p)
+                s64.reset()
+                s64.run(nt)
+                d[m] = s64.traces_f64()
+            fd = math.fsum(((d[1] - d[-1]) * (0.5 * (d[1] + d[-1]) - obs64)).ravel()) / (2 * hstep)
+            e = abs(g[name][k, j, i] - fd) / abs(fd)
+            worst = max(worst, e)
+            checks.append({"cell": [i, j, k], "param": name, "fd": fd, "adjoint_fp32": float(g[name][k, j, i]),
+                           "rel": round(e, 5)})
+    s64.close()
+    best["fd_check"] = {"worst_rel": round(worst, 5), "points": checks,
+                        "note": "FP32 forward residual, FP32 adjoint, FP64 gradient sums against second-order "
+                                "central differences of the FP64 forward misfit (step 1e-3 of the value)"}
     best.update(workload="adjoint128: 128^3, order 4, PML 10, 1000 steps, checkpoint every 50, "
                          "FP32 wavefields, FP64 gradient sums, 512 receivers x (vx, vz, p)",
                 unit="ms/shot", note="ms_per_shot: HIP-event time of the checkpointed forward "
                 "plus recompute, adjoint sweep and FP64 gradient sums; wall adds misfit, "
-                "gradient download and the host chain rule to the six parameters")
+                "gradient download and the host chain rule to the six parameters",
+                ratio_to_forward=round(best["ms_per_shot"] / best["forward_checkpointed_ms"], 2))
     return best
 
 
@@ -283,6 +361,98 @@ def copy_calibration(mib):
                       "copy_ms": round(ms, 4), "copy_GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}))
 
 
+def step_roofline(r, bv, bt, bstep, upd, kernels):
+    """Roofline of one time step: the step's algorithmic bytes over the two stencil kernels'
+    mean launch durations (HIP events on the solver's stream), with the per-kernel figures and
+    the calibrated traffic of both kernels from the committed profile."""
+    tv, tt = r["kernel_ms_v"], r["kernel_ms_t"]
+    ach = bstep / ((tv + tt) * 1e-3) / 1e9
+    trv, trt = profiled_traffic(kernels[0]), profiled_traffic(kernels[1])
+    return {"bound": "hbm", "kernel": f"{kernels[0]} + {kernels[1]} (one time step)",
+            "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": (trv + trt) if trv is not None and trt is not None else None,
+            "traffic_source": os.path.relpath(PROFILE, ROOT),
+            "algorithmic_bytes_per_step": bstep, "bytes_per_cell_update": round(bstep / upd, 2),
+            "mean_step_kernel_ms": round(tv + tt, 5),
+            "kernels": {
+                "velocity": {"kernel": kernels[0], "mean_launch_ms": round(tv, 5), "algorithmic_bytes": bv,
+                             "achieved_GBs": round(bv / (tv * 1e-3) / 1e9, 1),
+                             "frac": round(bv / (tv * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": trv},
+                "stress": {"kernel": kernels[1], "mean_launch_ms": round(tt, 5), "algorithmic_bytes": bt,
+                           "achieved_GBs": round(bt / (tt * 1e-3) / 1e9, 1),
+                           "frac": round(bt / (tt * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": trt}}}
+
+
+KERNEL_TAG = {"fp32": "P32", "fp16": "P16", "fp16c": "P16C", "fp16t": "P16T", "fp16m": "P16M", "fp64": "P64"}
+
+
+def policy_line(cfg, mode, r, ref_value, steps, n):
+    v = n * steps / r["wall_s"] / 1e6
+    bv, bt, bstep, upd, _ = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"], r["widths"], mode)
+    tag = KERNEL_TAG[mode]
+    roof = step_roofline(r, bv, bt, bstep, upd, (f"k_velocity<{tag},{cfg['order']}>", f"k_stress<{tag},{cfg['order']}>"))
+    return {"value": round(v, 1), "unit": "Mcell-updates/s", "speedup_vs_fp32": round(v / ref_value, 3),
+            "ms_per_step": round(r["wall_s"] * 1e3 / steps, 4), "step_achieved_GBs": roof["achieved"],
+            "step_frac": roof["frac"], "bytes_per_cell_update": roof["bytes_per_cell_update"],
+            "kernel_ms_v": round(r["kernel_ms_v"], 5), "kernel_ms_t": round(r["kernel_ms_t"], 5),
+            "finite": r["finite"], "tile_bx_by_zchunk": r["shape"]}
+
+
+def perturbed512_section(steps, warmup, modes=("fp32", "fp16m", "fp16")):
+    """The 512 x 512 x 384 perturbed-medium grid (the FP16-storage workload): throughput of the
+    FP32 drop-in and the 16-bit policies over `steps` timed steps after `warmup`."""
+    from biot_amd import configs
+
+    t0 = time.time()
+    cfg = configs.perturbed512(nt=warmup + steps)
+    log(f"[bench] 512x512x384 medium built in {time.time() - t0:.0f} s")
+    n = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    out = {"workload": f"perturbed512: 512x512x384 cells, order 4, PML 20, layered medium with seeded "
+                       f"+-10 % Gaussian-correlated (50 m) perturbations of Kb, mu, phi, 20 Hz Ricker, "
+                       f"{len(cfg['rec'])} receivers x (vx, vz, p); {steps} timed steps after {warmup}",
+           "cells": n}
+    ref = None
+    for mode in modes:
+        r = run_precision(cfg, mode, steps, warmup)
+        line = policy_line(cfg, mode, r, ref or 1.0, steps, n)
+        if mode == "fp32":
+            ref = line["value"]
+            line.pop("speedup_vs_fp32")
+        out[mode] = line
+        log(f"[bench] 512x512x384 {mode}: {line['value']:.0f} Mcell/s"
+            + (f" ({line['speedup_vs_fp32']:.2f}x fp32)" if mode != "fp32" else ""))
+    return out
+
+
+def stream_overlap(cfg, nsteps, chunk=100):
+    """Wall time of nsteps with every receiver trace copied to the host: biot_run followed by
+    biot_read_traces, against biot_run_stream_traces (copies of each chunk of trace rows on a
+    side stream while the loop continues); the two outputs must agree bit for bit."""
+    import numpy as np
+
+    from biot_amd import configs
+
+    s = configs.make_solver(cfg, precision="fp32", nt=nsteps)
+    s.autotune(5)
+    s.run(20)
+    s.reset()
+    t0 = time.perf_counter()
+    s.run(nsteps)
+    a = s.traces()
+    plain = time.perf_counter() - t0
+    s.reset()
+    t0 = time.perf_counter()
+    b = s.run_stream_traces(nsteps, chunk)
+    streamed = time.perf_counter() - t0
+    same = bool(np.array_equal(a, b))
+    s.close()
+    return {"workload": f"{cfg['name']} FP32, {nsteps} steps, {len(cfg['rec'])} receivers x 3 components "
+                        f"({a.nbytes / 1e9:.2f} GB of traces)",
+            "run_then_read_s": round(plain, 3), "streamed_s": round(streamed, 3),
+            "hidden_s": round(plain - streamed, 3), "chunk_steps": chunk, "outputs_equal": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +461,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-512", action="store_true")
+    ap.add_argument("--steps-512", type=int, default=300)
+    ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-adjoint", action="store_true")
@@ -314,12 +487,11 @@ def main():
     n = cfg["nx"] * cfg["ny"] * cfg["nz"]
     r32 = run_precision(cfg, "fp32", args.steps, args.warmup)
     value = n * args.steps / r32["wall_s"] / 1e6
-    log(f"[bench] fp32: {value:.0f} Mcell/s, V {r32['kernel_ms_v']:.3f} ms T {r32['kernel_ms_t']:.3f} ms")
     bv, bt, bstep, upd, memb = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"],
                                                  r32["widths"], "fp32")
-    ach_v = bv / (r32["kernel_ms_v"] * 1e-3) / 1e9
-    ach_t = bt / (r32["kernel_ms_t"] * 1e-3) / 1e9
-    ach_step = bstep / ((r32["kernel_ms_v"] + r32["kernel_ms_t"]) * 1e-3) / 1e9
+    roof = step_roofline(r32, bv, bt, bstep, upd, ("k_velocity<P32,4>", "k_stress<P32,4>"))
+    log(f"[bench] fp32: {value:.0f} Mcell/s, V {r32['kernel_ms_v']:.3f} ms T {r32['kernel_ms_t']:.3f} ms, "
+        f"step {roof['frac']:.3f} of HBM peak")
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -338,38 +510,23 @@ def main():
                                f"(vx, vz, p), FP32 storage",
                    "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "order": 4, "npml": 20,
                    "dx_m": cfg["dx"], "dt_s": cfg["dt"], "tile_bx_by_zchunk": r32["shape"]},
-        "roofline": {"bound": "hbm", "kernel": "k_velocity<P32,4>",
-                     "achieved": round(ach_v, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach_v / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic("k_velocity<P32,4>"),
-                     "traffic_source": "profiles/stencil_kernels_layered256.md",
-                     "algorithmic_bytes_per_launch": bv,
-                     "mean_launch_ms": round(r32["kernel_ms_v"], 5)},
-        "stress_kernel": {"kernel": "k_stress<P32,4>", "achieved_GBs": round(ach_t, 1),
-                          "frac": round(ach_t / HBM_PEAK_GBS, 4), "algorithmic_bytes": bt,
-                          "mean_launch_ms": round(r32["kernel_ms_t"], 5)},
+        "roofline": roof,
         "stream_copy_GBs": copy_gbs,
-        "step_roofline": {"achieved_GBs": round(ach_step, 1), "frac": round(ach_step / HBM_PEAK_GBS, 4),
-                          "algorithmic_bytes_per_step": bstep,
-                          "bytes_per_cell_update": round(bstep / upd, 2)},
         "finite": r32["finite"],
     }
-    for mode in ([] if args.no_fp16 else ["fp16c", "fp16t", "fp16"]):
+    for mode in ([] if args.no_fp16 else ["fp16m", "fp16t", "fp16c", "fp16"]):
         r16 = run_precision(cfg, mode, args.steps, args.warmup)
-        v16 = n * args.steps / r16["wall_s"] / 1e6
-        _, _, bstep16, _, _ = algorithmic_bytes(cfg["nx"], cfg["ny"], cfg["nz"], cfg["order"],
-                                                r16["widths"], mode)
-        a16 = bstep16 / ((r16["kernel_ms_v"] + r16["kernel_ms_t"]) * 1e-3) / 1e9
-        log(f"[bench] {mode}: {v16:.0f} Mcell/s ({v16 / value:.2f}x fp32)")
-        out[mode] = {"value": round(v16, 1), "unit": "Mcell-updates/s",
-                     "speedup_vs_fp32": round(v16 / value, 3),
-                     "ms_per_step": round(r16["wall_s"] * 1e3 / args.steps, 4),
-                     "step_achieved_GBs": round(a16, 1),
-                     "step_frac": round(a16 / HBM_PEAK_GBS, 4),
-                     "bytes_per_cell_update": round(bstep16 / upd, 2),
-                     "kernel_ms_v": round(r16["kernel_ms_v"], 5),
-                     "kernel_ms_t": round(r16["kernel_ms_t"], 5), "finite": r16["finite"],
-                     "tile_bx_by_zchunk": r16["shape"]}
+        out[mode] = policy_line(cfg, mode, r16, value, args.steps, n)
+        log(f"[bench] {mode}: {out[mode]['value']:.0f} Mcell/s ({out[mode]['speedup_vs_fp32']:.2f}x fp32), "
+            f"step {out[mode]['step_frac']:.3f} of HBM peak")
+    if not args.no_512:
+        log("[bench] 512x512x384 ...")
+        out["perturbed512"] = perturbed512_section(args.steps_512, args.warmup)
+    if not args.no_stream:
+        log("[bench] streamed trace output ...")
+        out["stream_overlap"] = stream_overlap(cfg, args.steps)
+        log(f"[bench] streamed: {out['stream_overlap']['streamed_s']:.2f} s vs run+read "
+            f"{out['stream_overlap']['run_then_read_s']:.2f} s")
     if not args.no_adjoint:
         log("[bench] adjoint 128^3 x 1000 steps ...")
         out["adjoint"] = adjoint_shot()

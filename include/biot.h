@@ -51,8 +51,15 @@ enum biot_precision {
   BIOT_FP64 = 2,  /* double storage and arithmetic (accuracy reference on the GPU) */
   BIOT_FP16C = 3, /* as BIOT_FP16 with float C-PML memory, plus an 8-bit residual per value of
                      v, tau and p (value = h + r * ulp(h)/256): compensated half precision */
-  BIOT_FP16T = 4  /* as BIOT_FP16C with the residual on tau and p only (velocities plain
+  BIOT_FP16T = 4, /* as BIOT_FP16C with the residual on tau and p only (velocities plain
                      binary16): faster, with a larger error than BIOT_FP16C */
+  BIOT_FP16M = 5  /* mixed 16/24-bit storage: v, q binary16 (per-field scaling); tau, p 24-bit
+                     floats (the upper three bytes of a float: 8 exponent and 15 stored
+                     significand bits); the medium as 9 binary16 cell moduli (rho, rho_f, C1, C2,
+                     C, M, lambda_c, H, 1/mu) from which the kernels form the staggered averages
+                     and coefficients; float C-PML memory and arithmetic.  Needs the moduli:
+                     biot_set_medium, biot_set_medium_device or biot_set_params (not
+                     biot_set_coefficients); no adjoint or batches. */
 };
 
 enum biot_field {

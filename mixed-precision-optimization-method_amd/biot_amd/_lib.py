@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libbiot.so")
 BIOT_OK = 0
 STATUS = {0: "BIOT_OK", 1: "BIOT_ERR_ARG", 2: "BIOT_ERR_HIP", 3: "BIOT_ERR_STATE",
           4: "BIOT_ERR_NONFINITE", 5: "BIOT_ERR_RANGE", 6: "BIOT_ERR_NOMEM"}
-FP32, FP16, FP64, FP16C, FP16T = 0, 1, 2, 3, 4
+FP32, FP16, FP64, FP16C, FP16T, FP16M = 0, 1, 2, 3, 4, 5
 FIELDS = ["vx", "vy", "vz", "qx", "qy", "qz", "txx", "tyy", "tzz", "txy", "txz", "tyz", "p"]
 FIELD_INDEX = {n: i for i, n in enumerate(FIELDS)}
 NCOEF = 23

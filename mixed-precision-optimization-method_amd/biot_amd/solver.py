@@ -13,7 +13,7 @@ from ._lib import (BIOT_OK, FIELD_INDEX, FIELDS, NCOEF, BiotError, Grid, Medium,
                    i64ptr, lib, mask_of)
 
 PRECISION = {"fp32": _lib.FP32, "fp16": _lib.FP16, "fp64": _lib.FP64, "fp16c": _lib.FP16C,
-             "fp16t": _lib.FP16T}
+             "fp16t": _lib.FP16T, "fp16m": _lib.FP16M}
 MEDIUM_KEYS = ("Ks", "Kf", "Kb", "mu", "phi", "rhos", "rhof", "eta", "perm")
 
 

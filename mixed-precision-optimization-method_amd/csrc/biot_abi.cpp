@@ -52,13 +52,36 @@ double half_res_unit(_Float16 h)
   return std::ldexp(1.0, e - 33);
 }
 
+// FP16M's 24-bit float: the upper two bytes of the float in a 16-bit array, the third byte in
+// a byte array, rounded to nearest even on the dropped byte
+void w24_store(float x, uint16_t &hi, int8_t &lo)
+{
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  b = b + 0x7Fu + ((b >> 8) & 1u);
+  hi = (uint16_t)(b >> 16);
+  lo = (int8_t)(uint8_t)(b >> 8);
+}
+double w24_value(uint16_t hi, int8_t lo)
+{
+  const uint32_t b = ((uint32_t)hi << 16) | ((uint32_t)(uint8_t)lo << 8);
+  float f;
+  std::memcpy(&f, &b, 4);
+  return (double)f;
+}
+
 // host storage-type conversion; lo: the residuals of a compensated field, else null
 void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst, int8_t *lo)
 {
+  if (prec == BIOT_FP16M && lo) {
+    uint16_t *d = (uint16_t *)dst;
+    for (int64_t c = 0; c < n; ++c) w24_store((float)(src[c] * scale_inv), d[c], lo[c]);
+    return;
+  }
   if (prec == BIOT_FP32) {
     float *d = (float *)dst;
     for (int64_t c = 0; c < n; ++c) d[c] = (float)(src[c] * scale_inv);
-  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T) {
+  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T || prec == BIOT_FP16M) {
     _Float16 *d = (_Float16 *)dst;
     for (int64_t c = 0; c < n; ++c) {
       const float x = (float)(src[c] * scale_inv);
@@ -75,10 +98,15 @@ void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *
 }
 void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double scale, double *dst)
 {
+  if (prec == BIOT_FP16M && lo) {
+    const uint16_t *s = (const uint16_t *)src;
+    for (int64_t c = 0; c < n; ++c) dst[c] = w24_value(s[c], lo[c]) * scale;
+    return;
+  }
   if (prec == BIOT_FP32) {
     const float *s = (const float *)src;
     for (int64_t c = 0; c < n; ++c) dst[c] = (double)s[c] * scale;
-  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T) {
+  } else if (prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T || prec == BIOT_FP16M) {
     const _Float16 *s = (const _Float16 *)src;
     for (int64_t c = 0; c < n; ++c) {
       double v = (double)(float)s[c];
@@ -139,8 +167,8 @@ static int validate_grid(const biot_grid *g, std::string &why)
     why = "grid too large";
     return BIOT_ERR_ARG;
   }
-  if (g->precision < 0 || g->precision > 4) {
-    why = "precision must be BIOT_FP32, BIOT_FP16, BIOT_FP64, BIOT_FP16C or BIOT_FP16T";
+  if (g->precision < 0 || g->precision > 5) {
+    why = "precision must be BIOT_FP32, BIOT_FP16, BIOT_FP64, BIOT_FP16C, BIOT_FP16T or BIOT_FP16M";
     return BIOT_ERR_ARG;
   }
   if (!(g->dx > 0) || !(g->dt > 0) || !(g->Cv > 0) || !(g->Cp > 0)) {
@@ -205,6 +233,8 @@ extern "C" int biot_create(const biot_grid *g, biot_handle **out)
   const int64_t lim = (int64_t)1 << 32;
   if (h->n * (int64_t)h->esz >= lim)
     return refuse(h, BIOT_ERR_ARG, "grid too large: one field array would exceed 4 GiB");
+  if (h->prec == BIOT_FP16M && h->n * 8 >= lim)
+    return refuse(h, BIOT_ERR_ARG, "grid too large: an FP16M medium record array would exceed 4 GiB");
   for (int ax = 0; ax < 3; ++ax)
     if (NCARRY * h->slab[ax] * (int64_t)h->csz >= lim)
       return refuse(h, BIOT_ERR_ARG, "absorbing slabs too large: one axis's C-PML memory would exceed 4 GiB");
@@ -266,6 +296,10 @@ extern "C" int biot_set_medium(biot_handle *h, const biot_medium *m)
   build_moduli(h->n, *m, md);
   build_averages(h->g.nx, h->g.ny, h->g.nz, md, av);
   build_coefficients(h->g.nx, h->g.ny, h->g.nz, h->g.dt, h->g.Cv, h->g.Cp, md, av, h->coef_ref);
+  if (h->prec == BIOT_FP16M) {
+    std::vector<float> *src[9] = {&md.M, &md.HH, &md.H2u, &md.C, &md.C1, &md.C2, &md.rho, &md.rhof, &md.mu};
+    for (int q = 0; q < 9; ++q) h->mod[q].swap(*src[q]);
+  }
   return h->upload_coefficients();
 }
 
@@ -318,6 +352,13 @@ extern "C" int biot_set_medium_device(biot_handle *h, const biot_medium *m)
         hipSuccess)
       rc = BIOT_ERR_HIP;
   }
+  // FP16M builds its medium records from the moduli (M HH H2u C C1 C2 rho rhof mu)
+  for (int q = 0; q < 9 && !rc && h->prec == BIOT_FP16M; ++q) {
+    h->mod[q].resize((size_t)n);
+    if (hipMemcpy(h->mod[q].data(), d_mod + (size_t)q * n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      rc = BIOT_ERR_HIP;
+  }
   if (rc) {
     for (void *p : {(void *)d_med, (void *)d_mod, (void *)d_ref}) (void)hipFree(p);
     return fail(h, rc, "device set-up failed");
@@ -352,6 +393,8 @@ extern "C" int biot_read_kernel_coefficients(biot_handle *h, int32_t k, double *
   if (!h || !out) return BIOT_ERR_ARG;
   if (k < 0 || k >= NKC) return fail(h, BIOT_ERR_ARG, "kernel coefficient %d out of range", k);
   if (!h->have_coef) return fail(h, BIOT_ERR_STATE, "no coefficients set");
+  if (h->prec == BIOT_FP16M)
+    return fail(h, BIOT_ERR_STATE, "FP16M handles store the medium moduli, not the coefficient arrays");
   std::vector<unsigned char> buf((size_t)h->n * h->esz);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(buf.data(), (const char *)h->d_coef + (size_t)k * h->n * h->esz, buf.size(),
@@ -395,6 +438,9 @@ extern "C" int biot_set_coefficients(biot_handle *h, const float *const coef[BIO
   if (!coef) return fail(h, BIOT_ERR_ARG, "coef is null");
   for (int a = 0; a < BIOT_NCOEF; ++a)
     if (!coef[a] && a != BIOT_W3Z) return fail(h, BIOT_ERR_ARG, "coefficient array %d is null", a);
+  if (h->prec == BIOT_FP16M)
+    return fail(h, BIOT_ERR_STATE,
+                "FP16M needs the medium moduli (biot_set_medium, biot_set_medium_device or biot_set_params)");
   h->forget_model();
   for (int a = 0; a < BIOT_NCOEF; ++a) {
     if (coef[a])

@@ -63,6 +63,23 @@ extern "C" int biot_set_params(biot_handle *h, const biot_params *p)
   }
   h->free_adjoint();  // checkpoints of the previous model are stale
   for (int q = 0; q < 8; ++q) h->par[q].assign(src[q], src[q] + h->n);
+  if (h->prec == BIOT_FP16M) {
+    // the cell moduli of the parameters (outputfunc.cpp:210-217 in parameter form): C = alpha M,
+    // lambda_c = lambda + alpha^2 M, H = lambda_c + 2 mu, C2 = tphi rho_f
+    for (auto &v : h->mod) v.resize((size_t)h->n);
+    for (int64_t c = 0; c < h->n; ++c) {
+      const double al = p->alpha[c], M = p->M[c], lc = p->lambda[c] + al * al * M;
+      h->mod[0][c] = (float)M;
+      h->mod[1][c] = (float)(lc + 2.0 * p->mu[c]);
+      h->mod[2][c] = (float)lc;
+      h->mod[3][c] = (float)(al * M);
+      h->mod[4][c] = (float)p->C1[c];
+      h->mod[5][c] = (float)(p->tphi[c] * p->rhof[c]);
+      h->mod[6][c] = (float)p->rho[c];
+      h->mod[7][c] = (float)p->rhof[c];
+      h->mod[8][c] = (float)p->mu[c];
+    }
+  }
   kernel_coefs_from_params(h->g.nx, h->g.ny, h->g.nz, coef_scales(h), param_view(h), h->K_last);
   for (auto &v : h->coef_ref) std::vector<float>().swap(v);  // the reference arrays no longer apply
   return h->upload_kernel_coefs(h->K_last);
@@ -71,7 +88,7 @@ extern "C" int biot_set_params(biot_handle *h, const biot_params *p)
 extern "C" int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every)
 {
   if (!h) return BIOT_ERR_ARG;
-  if (h->prec == BIOT_FP16C || h->prec == BIOT_FP16T)
+  if (h->prec == BIOT_FP16C || h->prec == BIOT_FP16T || h->prec == BIOT_FP16M)
     return afail(h, BIOT_ERR_ARG, "the adjoint path runs on FP32, FP64 or FP16 handles");
   if (nt <= 0 || every <= 0) return afail(h, BIOT_ERR_ARG, "nt and every must be positive");
   if (!h->have_coef) return afail(h, BIOT_ERR_STATE, "no medium set");

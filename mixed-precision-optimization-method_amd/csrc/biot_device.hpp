@@ -26,6 +26,13 @@
 //   FP16T as FP16C with the residual on tau and p only (the velocities plain binary16): the
 //         stresses' rounding dominates the binary16 error, and the velocity residuals are
 //         a third of FP16C's residual traffic.
+//   FP16M v and q binary16 (scaled); tau and p 24-bit floats, the upper three bytes of a
+//         float (sign, 8 exponent bits, 15 stored significand bits, rounded to nearest even),
+//         kept as a 16-bit array of the upper two bytes and the int8 array of the third; the
+//         medium as three per-cell records of binary16 moduli, scaled to [-1, 1] (mv: rho,
+//         rho_f, C1, C2; mt: C, M, lambda_c, H; mr: 1/mu, +inf where mu = 0), from which the
+//         kernels form the staggered averages and the 22 coefficients in registers; float
+//         C-PML carries.
 //   FP64  everything double.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -35,18 +42,30 @@
 namespace biot {
 
 enum { NFIELD = 13, NKC = 22, NCARRY = 8 };
-enum { PREC_FP32 = 0, PREC_FP16 = 1, PREC_FP64 = 2, PREC_FP16C = 3, PREC_FP16T = 4 };
+enum { PREC_FP32 = 0, PREC_FP16 = 1, PREC_FP64 = 2, PREC_FP16C = 3, PREC_FP16T = 4, PREC_FP16M = 5 };
 // fields carrying a residual: FP16C vx vy vz txx tyy tzz txy txz tyz p; FP16T txx .. tyz p
 constexpr uint32_t COMP_FIELDS_C = 0x1FC7u, COMP_FIELDS_T = 0x1FC0u;
 constexpr uint32_t comp_fields(int prec)
 {
-  return prec == PREC_FP16C ? COMP_FIELDS_C : prec == PREC_FP16T ? COMP_FIELDS_T : 0u;
+  return prec == PREC_FP16C ? COMP_FIELDS_C : (prec == PREC_FP16T || prec == PREC_FP16M) ? COMP_FIELDS_T : 0u;
 }
 // kernel coefficient order
 enum {
   K_U1X = 0, K_U1Y, K_U1Z, K_U2X, K_U2Y, K_U2Z, K_W1X, K_W1Y, K_W1Z, K_W2X, K_W2Y, K_W2Z,
   K_W3X, K_W3Y,
   K_P1, K_P2, K_S1, K_S2, K_S3, K_SXY, K_SXZ, K_SYZ
+};
+
+// FP16M: the constants that turn the stored moduli into kernel coefficients (in the storage
+// units of the fields they connect, 1/dx folded in).  Averages along an axis are
+// h * (stored(c) + stored(c + axis)); then U1 = cU1 / rho, U2 = kU2 rho_f / rho,
+// m = C2 - rho_f^2 / rho, a = aH C1, W1 = (m - a) / (m + a), W2 = cW2 (rho_f / rho) / (m + a),
+// W3 = cW3 / (m + a) (main.cpp:447-481); P1 = kP1 C, P2 = kP2 M, S1 = kS1 lambda_c, S2 = kS2 H,
+// S3 = kS3 C, Sij = kSh / (sum of the stored 1/mu over the 4-cell quad) (main.cpp:326-337,
+// 483-494), all on the stored values.
+enum {
+  MC_HRHO = 0, MC_HRHOF, MC_HC1, MC_HC2, MC_KU2, MC_CU1, MC_CW2, MC_CW3, MC_AH,
+  MC_KP1, MC_KP2, MC_KS1, MC_KS2, MC_KS3, MC_KSH, NMC
 };
 
 // Kernel arguments; pointers are type-erased, their element types follow the storage mode.
@@ -59,6 +78,9 @@ struct StepArgs {
   // in 64 bits: these arrays may span more than 4 GiB)
   const void *kv, *kt;
   float ks[NKC];             // per-array scale (binary16 storage), 1 otherwise
+  // FP16M: the medium records (mv, mt: 4 binary16 values per cell; mr: one) and constants
+  const void *mv, *mt, *mr;
+  float mc[NMC];
   // per axis: the C-PML carries (carry type) of the slab cells, in two groups of records — the
   // velocity half step's carries 0..3, then the stress half step's 4..7 — each group slab[ax]
   // records of 4 carries: carry m of slab cell o is element (m/4)*4*slab + 4*o + m%4

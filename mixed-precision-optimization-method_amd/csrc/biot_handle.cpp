@@ -3,9 +3,11 @@
 #include "biot_setup.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 
 using namespace biot;
 
@@ -211,6 +213,7 @@ void biot_handle::free_adjoint()
 void biot_handle::forget_model()
 {
   free_adjoint();
+  for (auto &v : mod) std::vector<float>().swap(v);
   for (auto &v : par) std::vector<double>().swap(v);
   for (auto &v : K_last) std::vector<double>().swap(v);
 }
@@ -258,7 +261,14 @@ void biot_handle::fill_args(StepArgs &a) const
     a.ks[k] = ks[k];
   }
   a.kv = d_kpack;
-  a.kt = (const char *)d_kpack + (size_t)14 * n * esz;
+  a.kt = d_kpack ? (const char *)d_kpack + (size_t)14 * n * esz : nullptr;
+  if (prec == BIOT_FP16M) {
+    for (int k = 0; k < NKC; ++k) a.k[k] = nullptr;
+    a.mv = d_coef;
+    a.mt = (const char *)d_coef + (size_t)8 * n;
+    a.mr = (const char *)d_coef + (size_t)16 * n;
+    for (int q = 0; q < NMC; ++q) a.mc[q] = mc[q];
+  }
   for (int ax = 0; ax < 3; ++ax) {
     a.carry[ax] = d_carry[ax];
     a.slab[ax] = slab[ax];
@@ -291,9 +301,11 @@ int biot_handle::allocate()
     device_bytes += (int64_t)bytes;
     return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
   };
-  bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, (size_t)NKC * n * esz) &&
-            alloc(&d_kpack, (size_t)NKC * n * esz) && alloc((void **)&d_flag, 256) &&
-            clear_nonfinite() == BIOT_OK;
+  // FP16M keeps the medium records (18 bytes per cell) in place of the coefficient arrays
+  const size_t coef_bytes = prec == BIOT_FP16M ? (size_t)18 * n : (size_t)NKC * n * esz;
+  const size_t pack_bytes = prec == BIOT_FP16M ? 0 : (size_t)NKC * n * esz;
+  bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, coef_bytes) &&
+            alloc(&d_kpack, pack_bytes) && alloc((void **)&d_flag, 256) && clear_nonfinite() == BIOT_OK;
   if (ok && comp_fields(prec)) ok = alloc((void **)&d_lo, (size_t)NFIELD * n);
   for (int ax = 0; ax < 3 && ok; ++ax) ok = alloc(&d_carry[ax], (size_t)NCARRY * slab[ax] * csz);
   const int dims[3] = {g.nx, g.ny, g.nz};
@@ -557,6 +569,11 @@ int biot_handle::upload_from_ref(const float *d_ref)
     ra.ratio[k] = kfac[k];
     ra.inv_ks[k] = 1.0 / (double)ks[k];
   }
+  if (prec == BIOT_FP16M) {
+    if ((rc = upload_medium())) return rc;
+    have_coef = true;
+    return BIOT_OK;
+  }
   if (launch_ref_to_kernel(prec, ra, stream) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) {
     err = "coefficient conversion failed";
     return BIOT_ERR_HIP;
@@ -596,6 +613,11 @@ int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
   set_scales(amaxK, resp);
   int rc = rescale_state(old_scale);
   if (rc) return rc;
+  if (prec == BIOT_FP16M) {
+    if ((rc = upload_medium())) return rc;
+    have_coef = true;
+    return BIOT_OK;
+  }
   std::vector<double> K[NKC];
   for (int k = 0; k < NKC; ++k) {
     K[k] = K_in[k];
@@ -628,5 +650,100 @@ int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
   }
   if ((rc = pack_coefficients())) return rc;
   have_coef = true;
+  return BIOT_OK;
+}
+
+// FP16M: the medium records from the 9 cell moduli.  Each stored quantity is scaled by its
+// largest magnitude (1/mu by its largest finite value; a zero mu stores +inf), rounded to
+// binary16; the kernels' constants fold the scales back together with the reference's
+// coefficient factors (Cv, Cp, DT, 1/dx as the kernels hold it, the 0.01 x 100 of the
+// stress update) and the field-storage ratios kfac.
+namespace {
+enum { MD_M = 0, MD_HH, MD_H2U, MD_C, MD_C1, MD_C2, MD_RHO, MD_RHOF, MD_MU };
+struct MedCtx {
+  const std::vector<float> *mod;
+  double inv[9];       // 1 / scale per record slot: mv rho rhof C1 C2, mt C M H2u HH, mr 1/mu
+  uint16_t *mv, *mt, *mr;
+  double mx[64][9];    // per-thread maxima (first pass)
+  std::atomic<int> slot{0};
+};
+inline uint16_t h16(double x)
+{
+  const _Float16 h = (_Float16)(float)x;
+  return __builtin_bit_cast(uint16_t, h);
+}
+const int MED_SRC[9] = {MD_RHO, MD_RHOF, MD_C1, MD_C2, MD_C, MD_M, MD_H2U, MD_HH, MD_MU};
+void med_max_body(int64_t lo, int64_t hi, void *p)
+{
+  auto *cx = static_cast<MedCtx *>(p);
+  double m[9] = {};
+  for (int64_t c = lo; c < hi; ++c)
+    for (int q = 0; q < 9; ++q) {
+      const double v = cx->mod[MED_SRC[q]][c];
+      const double a = q == 8 ? (v > 0 ? 1.0 / v : 0.0) : std::fabs(v);
+      m[q] = std::max(m[q], a);
+    }
+  const int t = cx->slot.fetch_add(1);
+  for (int q = 0; q < 9; ++q) cx->mx[t][q] = m[q];
+}
+void med_pack_body(int64_t lo, int64_t hi, void *p)
+{
+  auto *cx = static_cast<MedCtx *>(p);
+  const std::vector<float> *M = cx->mod;
+  for (int64_t c = lo; c < hi; ++c) {
+    for (int q = 0; q < 4; ++q) {
+      cx->mv[4 * c + q] = h16(M[MED_SRC[q]][c] * cx->inv[q]);
+      cx->mt[4 * c + q] = h16(M[MED_SRC[4 + q]][c] * cx->inv[4 + q]);
+    }
+    const double mu = M[MD_MU][c];
+    cx->mr[c] = mu > 0 ? h16(1.0 / mu * cx->inv[8]) : h16(INFINITY);
+  }
+}
+}  // namespace
+
+int biot_handle::upload_medium()
+{
+  for (const auto &v : mod)
+    if ((int64_t)v.size() != n) {
+      err = "FP16M needs the medium moduli (biot_set_medium, biot_set_medium_device or biot_set_params)";
+      return BIOT_ERR_STATE;
+    }
+  auto cx = std::make_unique<MedCtx>();
+  cx->mod = mod;
+  std::memset(cx->mx, 0, sizeof cx->mx);
+  parallel_for(n, med_max_body, cx.get());
+  double sc[9] = {};
+  for (int t = 0; t < 64; ++t)
+    for (int q = 0; q < 9; ++q) sc[q] = std::max(sc[q], cx->mx[t][q]);
+  for (int q = 0; q < 9; ++q) {
+    if (!(sc[q] > 0) || !std::isfinite(sc[q])) sc[q] = 1.0;
+    cx->inv[q] = 1.0 / sc[q];
+  }
+  std::vector<uint16_t> rec((size_t)9 * n);
+  cx->mv = rec.data();
+  cx->mt = rec.data() + (size_t)4 * n;
+  cx->mr = rec.data() + (size_t)8 * n;
+  parallel_for(n, med_pack_body, cx.get());
+  if (hipMemcpy(d_coef, rec.data(), rec.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess) {
+    err = "medium upload failed";
+    return BIOT_ERR_HIP;
+  }
+  const double Hinv = (double)(1.0f / g.dx), DT = (double)g.dt, Cv = (double)g.Cv, Cp = (double)g.Cp;
+  const double vel = Cv * DT / Cp * Hinv, str = Cp * DT / Cv * Hinv;
+  mc[MC_HRHO] = (float)(0.5 * sc[0]);
+  mc[MC_HRHOF] = (float)(0.5 * sc[1]);
+  mc[MC_HC1] = (float)(0.5 * sc[2]);
+  mc[MC_HC2] = (float)(0.5 * sc[3]);
+  mc[MC_KU2] = (float)kfac[K_U2X];
+  mc[MC_CU1] = (float)(vel * kfac[K_U1X]);
+  mc[MC_CW2] = (float)(vel * kfac[K_W2X]);
+  mc[MC_CW3] = (float)(vel * kfac[K_W3X]);
+  mc[MC_AH] = (float)(0.5 * DT);
+  mc[MC_KP1] = (float)(sc[4] * str * kfac[K_P1]);
+  mc[MC_KP2] = (float)(sc[5] * str * kfac[K_P2]);
+  mc[MC_KS1] = (float)(sc[6] * str * kfac[K_S1]);
+  mc[MC_KS2] = (float)(sc[7] * str * kfac[K_S2]);
+  mc[MC_KS3] = (float)(sc[4] * str * kfac[K_S3]);
+  mc[MC_KSH] = (float)(4.0 * str * kfac[K_SXY] / sc[8]);
   return BIOT_OK;
 }

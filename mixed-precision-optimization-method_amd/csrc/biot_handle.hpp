@@ -23,7 +23,7 @@ struct biot_handle {
   // device memory
   void *d_fields = nullptr;          // 13 * n
   int8_t *d_lo = nullptr;            // FP16C / FP16T residuals, 13 * n (compensated fields used)
-  void *d_coef = nullptr;            // 22 * n
+  void *d_coef = nullptr;            // 22 * n (FP16M: the medium records, 18 bytes per cell)
   void *d_kpack = nullptr;           // the coefficients packed per cell (velocity, then stress block)
   void *d_carry[3] = {nullptr, nullptr, nullptr};
   int *d_flag = nullptr;             // first step with a non-finite receiver sample
@@ -40,6 +40,10 @@ struct biot_handle {
   float ks[biot::NKC];        // per-coefficient-array scale (binary16 storage)
   double kfac[biot::NKC];     // storage-unit coefficient / physical coefficient, per array
   float fscale[biot::NFIELD]; // per-field storage scale
+  // FP16M: the 9 cell moduli the medium records are built from (M HH H2u C C1 C2 rho rhof mu,
+  // float, as biot::Moduli) and the constants of the kernels' coefficient formulas
+  std::vector<float> mod[9];
+  float mc[biot::NMC] = {};
 
   // source
   std::vector<float> wavelet;
@@ -63,7 +67,10 @@ struct biot_handle {
   ~biot_handle();
   void *field_ptr(int f) const;
   int8_t *lo_ptr(int f) const;  // residual array of field f, or null
-  bool half() const { return prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T; }
+  bool half() const
+  {
+    return prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T || prec == BIOT_FP16M;
+  }
   void fill_args(biot::StepArgs &a) const;
   int allocate();
   int upload_profiles();
@@ -71,6 +78,7 @@ struct biot_handle {
   int pack_coefficients();           // d_coef -> d_kpack
   int upload_kernel_coefs(const std::vector<double> (&K)[biot::NKC]);
   int upload_from_ref(const float *d_ref);      // 23 reference arrays on the device
+  int upload_medium();                          // FP16M: mod -> medium records, mc
   void set_scales(const double amaxK[biot::NKC], double resp);
   int rescale_state(const float old_scale[biot::NFIELD]);
 

@@ -26,10 +26,11 @@ namespace {
 
 using u32 = uint32_t;
 
-// cmask: the fields that carry a residual; nt_v / nt_t: the velocity / stress kernel's
-// own-value loads and field stores are nontemporal (same-box A/B: FP32 velocity kernel
-// -3.5 % time, FP16 velocity kernel -1 % step time; the residual policies 2-3 % slower with
-// them, FP16's stress kernel neutral)
+// cmask: the fields that carry a residual (w24: the 24-bit float form instead); nt_v / nt_t:
+// the velocity / stress kernel's own-value loads and field stores are nontemporal (same-box
+// A/B: FP32 velocity kernel -3.5 % time, FP16 velocity kernel -1 % step time; the residual
+// policies 2-3 % slower with them, FP16's stress kernel neutral); medium: the coefficients
+// are formed from the stored moduli (FP16M) instead of read from the packed arrays
 struct P32 {
   using S = float;
   using SC = float;
@@ -37,6 +38,7 @@ struct P32 {
   static constexpr bool comp = false, scaled = false;
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = true, nt_t = true;
+  static constexpr bool w24 = false, medium = false;
 };
 struct P16 {
   using S = _Float16;
@@ -45,6 +47,7 @@ struct P16 {
   static constexpr bool comp = false, scaled = true;
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = true, nt_t = false;
+  static constexpr bool w24 = false, medium = false;
 };
 struct P16C {
   using S = _Float16;
@@ -53,6 +56,7 @@ struct P16C {
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_C;
   static constexpr bool nt_v = false, nt_t = false;
+  static constexpr bool w24 = false, medium = false;
 };
 struct P16T {
   using S = _Float16;
@@ -61,6 +65,18 @@ struct P16T {
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_T;
   static constexpr bool nt_v = false, nt_t = false;
+  static constexpr bool w24 = false, medium = false;
+};
+// v, q binary16; tau, p 24-bit floats (upper 16 bits in the field array, the next 8 in the
+// residual array); coefficients from the stored moduli
+struct P16M {
+  using S = _Float16;
+  using SC = float;
+  using A = float;
+  static constexpr bool comp = true, scaled = true;
+  static constexpr uint32_t cmask = COMP_FIELDS_T;
+  static constexpr bool nt_v = false, nt_t = false;
+  static constexpr bool w24 = true, medium = true;
 };
 struct P64 {
   using S = double;
@@ -69,6 +85,7 @@ struct P64 {
   static constexpr bool comp = false, scaled = false;
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = false, nt_t = false;
+  static constexpr bool w24 = false, medium = false;
 };
 
 // element idx of an array, byte offset formed in 32 bits
@@ -93,15 +110,31 @@ __device__ __forceinline__ int h_exp(_Float16 h)
 __device__ __forceinline__ float res_unit(int e) { return __builtin_bit_cast(float, (u32)(e + 94) << 23); }
 __device__ __forceinline__ float res_inv(int e) { return __builtin_bit_cast(float, (u32)(160 - e) << 23); }
 
-// value = stored part (+ residual for the compensated fields of the FP16C / FP16T policies)
+// 24-bit float: the upper two bytes of the float's bits in a 16-bit array, the third byte in
+// a byte array; rounded to nearest even on the dropped low byte (overflow rounds to inf)
+__device__ __forceinline__ float w24_load(uint16_t hi, uint8_t lo)
+{
+  return __builtin_bit_cast(float, ((u32)hi << 16) | ((u32)lo << 8));
+}
+__device__ __forceinline__ u32 w24_round(float x)
+{
+  const u32 b = __builtin_bit_cast(u32, x);
+  return b + 0x7Fu + ((b >> 8) & 1u);
+}
+
+// value = stored part (+ residual for the compensated fields of the FP16C / FP16T policies;
+// the 24-bit form for FP16M's tau and p)
 template <class P>
 __device__ __forceinline__ typename P::A load_raw(const void *f, const int8_t *lo, bool comp_f,
                                                   u32 idx)
 {
   using S = typename P::S;
+  if constexpr (P::w24) {
+    if (comp_f) return w24_load(at<uint16_t>(f, idx), at<uint8_t>(lo, idx));
+  }
   const S h = at<S>(f, idx);
   typename P::A v = (typename P::A)h;
-  if constexpr (P::comp) {
+  if constexpr (P::comp && !P::w24) {
     if (comp_f) v += (float)at<int8_t>(lo, idx) * res_unit(h_exp(h));
   }
   return v;
@@ -111,11 +144,14 @@ template <class P, bool NT>
 __device__ __forceinline__ typename P::A load_own(const void *f, const int8_t *lo, bool comp_f, u32 idx)
 {
   using S = typename P::S;
+  if constexpr (P::w24) {
+    if (comp_f) return w24_load(at<uint16_t>(f, idx), at<uint8_t>(lo, idx));
+  }
   S h;
   if constexpr (NT) h = __builtin_nontemporal_load(&at<S>(f, idx));
   else h = at<S>(f, idx);
   typename P::A v = (typename P::A)h;
-  if constexpr (P::comp) {
+  if constexpr (P::comp && !P::w24) {
     if (comp_f) v += (float)at<int8_t>(lo, idx) * res_unit(h_exp(h));
   }
   return v;
@@ -125,10 +161,18 @@ __device__ __forceinline__ void store_raw(void *f, int8_t *lo, bool comp_f, u32 
                                           typename P::A x)
 {
   using S = typename P::S;
+  if constexpr (P::w24) {
+    if (comp_f) {
+      const u32 r = w24_round(x);
+      at<uint16_t>(f, idx) = (uint16_t)(r >> 16);
+      at<uint8_t>(lo, idx) = (uint8_t)(r >> 8);
+      return;
+    }
+  }
   const S h = (S)x;
   if constexpr (NT) __builtin_nontemporal_store(h, &at<S>(f, idx));
   else at<S>(f, idx) = h;
-  if constexpr (P::comp) {
+  if constexpr (P::comp && !P::w24) {
     if (comp_f) {
       const float r = rintf((x - (float)h) * res_inv(h_exp(h)));
       at<int8_t>(lo, idx) = (int8_t)fminf(fmaxf(r, -127.0f), 127.0f);
@@ -208,6 +252,80 @@ __device__ __forceinline__ void coef_block(const AT &a, const void *base, u32 c,
     if constexpr (P::scaled) out[e] = (A)packed::value<S>(d, e) * a.ks[k0 + e];
     else out[e] = (A)packed::value<S>(d, e);
   }
+}
+
+// FP16M medium record of cell c: 4 binary16 values in 8 bytes
+__device__ __forceinline__ void med_rec(const void *base, u32 c, float (&v)[4])
+{
+  typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
+  const dw2_t d = *reinterpret_cast<const dw2_t *>(reinterpret_cast<const char *>(base) + c * 8u);
+  v[0] = (float)__builtin_bit_cast(_Float16, (uint16_t)d.x);
+  v[1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(d.x >> 16));
+  v[2] = (float)__builtin_bit_cast(_Float16, (uint16_t)d.y);
+  v[3] = (float)__builtin_bit_cast(_Float16, (uint16_t)(d.y >> 16));
+}
+__device__ __forceinline__ float med_r(const void *base, u32 c) { return (float)at<_Float16>(base, c); }
+
+// The velocity half step's 14 coefficients (packed order K_U1X..K_W3Y) of cell c from the
+// FP16M moduli records: per axis the staggered averages of rho, rho_f, C1, C2 over c and its
+// forward neighbour (main.cpp:339-353), then the coefficient formulas of main.cpp:447-481
+// with their zero guards (every W when a C1 average is zero, every U when a rho average is).
+template <class AT>
+__device__ __forceinline__ void medium_velocity(const AT &a, u32 c, u32 sy, u32 sz, float (&kv)[14])
+{
+  float m0[4], mn[3][4];
+  med_rec(a.mv, c, m0);
+  med_rec(a.mv, c + 1u, mn[0]);
+  med_rec(a.mv, c + sy, mn[1]);
+  med_rec(a.mv, c + sz, mn[2]);
+  bool zw = false, zu = false;
+  float W3[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float rho = a.mc[MC_HRHO] * (m0[0] + mn[d][0]);
+    const float rhof = a.mc[MC_HRHOF] * (m0[1] + mn[d][1]);
+    const float c1 = a.mc[MC_HC1] * (m0[2] + mn[d][2]);
+    const float c2 = a.mc[MC_HC2] * (m0[3] + mn[d][3]);
+    zw = zw || c1 == 0.0f;
+    zu = zu || rho == 0.0f;
+    const float rr = __builtin_amdgcn_rcpf(rho);
+    const float ratio = rhof * rr;
+    const float m = c2 - rhof * ratio, am = a.mc[MC_AH] * c1;
+    const float rD = __builtin_amdgcn_rcpf(m + am);
+    kv[K_U1X + d] = a.mc[MC_CU1] * rr;
+    kv[K_U2X + d] = a.mc[MC_KU2] * ratio;
+    kv[K_W1X + d] = (m - am) * rD;
+    kv[K_W2X + d] = a.mc[MC_CW2] * ratio * rD;
+    W3[d] = a.mc[MC_CW3] * rD;
+  }
+  kv[K_W3X] = W3[0];
+  kv[K_W3Y] = W3[1];
+#pragma unroll
+  for (int k = K_W1X; k <= K_W3Y; ++k) kv[k] = zw ? 0.0f : kv[k];
+#pragma unroll
+  for (int k = K_U1X; k <= K_U2Z; ++k) kv[k] = zu ? 0.0f : kv[k];
+}
+
+// The stress half step's 8 coefficients (K_P1..K_SYZ) of cell c: P1, P2, S1, S2, S3 from the
+// cell's C, M, lambda_c, H (main.cpp:483-490), the shear ones from the harmonic mean of mu
+// over each 4-cell quad (main.cpp:326-337), formed from the stored 1/mu: a zero mu is +inf
+// there, so the quad's coefficient is 0 as the reference sets it.
+template <class AT>
+__device__ __forceinline__ void medium_stress(const AT &a, u32 c, u32 sy, u32 sz, float (&kt)[8])
+{
+  float t[4];
+  med_rec(a.mt, c, t);
+  const float r0 = med_r(a.mr, c), rx = med_r(a.mr, c + 1u), ry = med_r(a.mr, c + sy);
+  const float rxy = med_r(a.mr, c + 1u + sy), rz = med_r(a.mr, c + sz);
+  const float rxz = med_r(a.mr, c + 1u + sz), ryz = med_r(a.mr, c + sy + sz);
+  kt[0] = a.mc[MC_KP1] * t[0];
+  kt[1] = a.mc[MC_KP2] * t[1];
+  kt[2] = a.mc[MC_KS1] * t[2];
+  kt[3] = a.mc[MC_KS2] * t[3];
+  kt[4] = a.mc[MC_KS3] * t[0];
+  kt[5] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((r0 + rx) + (ry + rxy));
+  kt[6] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((r0 + rx) + (rz + rxz));
+  kt[7] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((r0 + ry) + (rz + ryz));
 }
 
 // One folded C-PML update on a carry held in a register: psi = carry + a D;
@@ -355,7 +473,8 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     const A qx0 = LO<P, FQX, (P::nt_v && NT)>(a, c), qy0 = LO<P, FQY, (P::nt_v && NT)>(a, c), qz0 = LO<P, FQZ, (P::nt_v && NT)>(a, c);
     const A vx0 = LO<P, FVX, (P::nt_v && NT)>(a, c), vy0 = LO<P, FVY, (P::nt_v && NT)>(a, c), vz0 = LO<P, FVZ, (P::nt_v && NT)>(a, c);
     A kv[14];  // K[0..13]: U1 xyz, U2 xyz, W1 xyz, W2 xyz, W3x, W3y
-    coef_block<P, 14>(a, a.kv, c, 0, kv);
+    if constexpr (P::medium) medium_velocity(a, c, sy, sz, kv);
+    else coef_block<P, 14>(a, a.kv, c, 0, kv);
     const A w1x = kv[K_W1X], w2x = kv[K_W2X], w3x = kv[K_W3X];
     const A w1y = kv[K_W1Y], w2y = kv[K_W2Y], w3y = kv[K_W3Y];
     const A w1z = kv[K_W1Z], w2z = kv[K_W2Z];
@@ -508,7 +627,8 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     const A tzz0 = LO<P, FTZZ, (P::nt_t && NT)>(a, c), txy0 = LO<P, FTXY, (P::nt_t && NT)>(a, c), tyz0 = LO<P, FTYZ, (P::nt_t && NT)>(a, c);
     const A txz0 = LO<P, FTXZ, (P::nt_t && NT)>(a, c);
     A kt[8];  // K[14..21]: P1, P2, S1, S2, S3, Sxy, Sxz, Syz
-    coef_block<P, 8>(a, a.kt, c, K_P1, kt);
+    if constexpr (P::medium) medium_stress(a, c, sy, sz, kt);
+    else coef_block<P, 8>(a, a.kt, c, K_P1, kt);
     const A s1 = kt[K_S1 - K_P1], s2 = kt[K_S2 - K_P1], s3 = kt[K_S3 - K_P1];
     const A kp1 = kt[0], kp2 = kt[K_P2 - K_P1];
     const A kxy = kt[K_SXY - K_P1], kyz = kt[K_SYZ - K_P1], kxz = kt[K_SXZ - K_P1];
@@ -693,6 +813,17 @@ __global__ void k_set_counter(int *itp, int v)
   if (threadIdx.x == 0) atomicExch(itp, v);
 }
 
+// In-place multiply of an FP16M 24-bit array (upper 16 bits in p, the next byte in lo).
+__global__ void k_rescale24(uint16_t *p, uint8_t *lo, int64_t n, double f)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const u32 r = w24_round(w24_load(p[i], lo[i]) * (float)f);
+    p[i] = (uint16_t)(r >> 16);
+    lo[i] = (uint8_t)(r >> 8);
+  }
+}
+
 // In-place multiply of a stored array (re-expresses values when a storage scale changes).
 template <typename S>
 __global__ void k_rescale(S *p, int8_t *lo, int64_t n, double f)
@@ -792,6 +923,7 @@ hipError_t dispatch_order(int prec, const StepArgs &a, LaunchShape sh, hipStream
     case PREC_FP16: return launch_step<P16, ORDER, VEL>(a, sh, s);
     case PREC_FP16C: return launch_step<P16C, ORDER, VEL>(a, sh, s);
     case PREC_FP16T: return launch_step<P16T, ORDER, VEL>(a, sh, s);
+    case PREC_FP16M: return launch_step<P16M, ORDER, VEL>(a, sh, s);
     case PREC_FP64: return launch_step<P64, ORDER, VEL>(a, sh, s);
   }
   return hipErrorInvalidValue;
@@ -893,6 +1025,7 @@ hipError_t launch_source(int prec, const SourceArgs &a, hipStream_t s)
     case PREC_FP16: hipLaunchKernelGGL(k_source<P16>, dim3(1), dim3(64), 0, s, a); break;
     case PREC_FP16C: hipLaunchKernelGGL(k_source<P16C>, dim3(1), dim3(64), 0, s, a); break;
     case PREC_FP16T: hipLaunchKernelGGL(k_source<P16T>, dim3(1), dim3(64), 0, s, a); break;
+    case PREC_FP16M: hipLaunchKernelGGL(k_source<P16M>, dim3(1), dim3(64), 0, s, a); break;
     case PREC_FP64: hipLaunchKernelGGL(k_source<P64>, dim3(1), dim3(64), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
@@ -912,6 +1045,12 @@ hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hi
     case PREC_FP16T:
       hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, lo, n, f);
       break;
+    case PREC_FP16M:
+      if (lo)
+        hipLaunchKernelGGL(k_rescale24, dim3(blocks), dim3(256), 0, s, (uint16_t *)p, (uint8_t *)lo, n, f);
+      else
+        hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, lo, n, f);
+      break;
     case PREC_FP64:
       hipLaunchKernelGGL(k_rescale<double>, dim3(blocks), dim3(256), 0, s, (double *)p, lo, n, f);
       break;
@@ -930,6 +1069,7 @@ hipError_t launch_slice_xz(int prec, const void *f, const int8_t *lo, bool comp,
     case PREC_FP16: hipLaunchKernelGGL(k_slice_xz<P16>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     case PREC_FP16C: hipLaunchKernelGGL(k_slice_xz<P16C>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     case PREC_FP16T: hipLaunchKernelGGL(k_slice_xz<P16T>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
+    case PREC_FP16M: hipLaunchKernelGGL(k_slice_xz<P16M>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     case PREC_FP64: hipLaunchKernelGGL(k_slice_xz<P64>, grid, block, 0, s, f, lo, comp, scale, j, nx, ny, out); break;
     default: return hipErrorInvalidValue;
   }
@@ -991,6 +1131,7 @@ hipError_t launch_record(int prec, const RecordArgs &a, hipStream_t s)
     case PREC_FP16: hipLaunchKernelGGL(k_record<P16>, grid, block, 0, s, a); break;
     case PREC_FP16C: hipLaunchKernelGGL(k_record<P16C>, grid, block, 0, s, a); break;
     case PREC_FP16T: hipLaunchKernelGGL(k_record<P16T>, grid, block, 0, s, a); break;
+    case PREC_FP16M: hipLaunchKernelGGL(k_record<P16M>, grid, block, 0, s, a); break;
     case PREC_FP64: hipLaunchKernelGGL(k_record<P64>, grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }

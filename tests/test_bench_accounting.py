@@ -37,14 +37,19 @@ def test_algorithmic_bytes_closed_form():
     coef_access = {"v": 14, "t": 8, "step": 22}
     carry_access = {"v": 8, "t": 8, "step": 16}
     residual_access = {"fp16c": {"v": 10 + 3, "t": 10 + 7, "step": 20},   # v, tau, p carry residuals
-                       "fp16t": {"v": 7, "t": 7 + 7, "step": 14}}         # tau, p carry residuals
+                       "fp16t": {"v": 7, "t": 7 + 7, "step": 14},         # tau, p carry residuals
+                       "fp16m": {"v": 7, "t": 7 + 7, "step": 14}}         # tau, p third bytes
+    # FP16M reads binary16 moduli records instead of coefficients: velocity half step rho,
+    # rho_f, C1, C2 (4 x 2 B); stress half step C, M, lambda_c, H (4 x 2 B) and 1/mu (2 B)
+    medium_bytes = {"fp16m": {"v": 4 * 2, "t": 4 * 2 + 2, "step": 4 * 2 + 4 * 2 + 2}}
     for mode, (wf, wk, wc, wr) in {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 2, 0), "fp16c": (2, 2, 4, 1),
-                                   "fp16t": (2, 2, 4, 1), "fp64": (8, 8, 8, 0)}.items():
+                                   "fp16t": (2, 2, 4, 1), "fp16m": (2, 2, 4, 1), "fp64": (8, 8, 8, 0)}.items():
         v, t, step, u, m = bench.algorithmic_bytes(nx, ny, nz, order, pml, mode)
         assert (u, m) == (upd, memb)
         for key, got in (("v", v), ("t", t), ("step", step)):
             ra = residual_access.get(mode, {}).get(key, 0)
-            want = upd * (field_access[key] * wf + coef_access[key] * wk + ra * wr) + memb * carry_access[key] * wc
+            kb = medium_bytes[mode][key] if mode in medium_bytes else coef_access[key] * wk
+            want = upd * (field_access[key] * wf + kb + ra * wr) + memb * carry_access[key] * wc
             assert got == want, (mode, key)
 
 
@@ -70,3 +75,23 @@ def test_host_cpu_record():
     assert set(h) == {"nproc", "usable_cpus", "cpu_model"}
     assert h["nproc"] >= 1 and 1 <= h["usable_cpus"] <= h["nproc"]
     assert h["cpu_model"] is None or isinstance(h["cpu_model"], str)
+
+
+def test_adjoint_bytes_closed_form():
+    """The adjoint sweep's byte model from its access lists: 4-byte values, per updated cell and
+    adjoint step 49 + 24 + 76 + 26 accesses in the four passes, 24 carry accesses per slab
+    membership; the recomputed forward step; a state save per step and a state load per
+    segment (every field of every cell and every C-PML carry, read and written); a flush per
+    segment (22 partial sums read and cleared, 22 FP64 sums read and written)."""
+    nx, ny, nz, order, nt, every = 20, 18, 16, 4, 37, 10
+    pml = (3, 3, 2, 4, 3, 2)
+    v, t, _, upd, memb = bench.algorithmic_bytes(nx, ny, nz, order, pml, "fp32")
+    n = nx * ny * nz
+    slab = (pml[0] + pml[1]) * ny * nz + nx * (pml[2] + pml[3]) * nz + nx * ny * (pml[4] + pml[5])
+    state = (13 * n + 8 * slab) * 4
+    nseg = 4
+    want = nseg * 2 * state + nt * (2 * state + v + t + (49 + 24 + 76 + 26) * 4 * upd + 24 * 4 * memb) \
+        + nseg * 22 * (4 + 4 + 8 + 8) * upd
+    total, per_step, u = bench.adjoint_bytes(nx, ny, nz, order, pml, nt, every)
+    assert u == upd and total == want
+    assert per_step == 2 * state + v + t + 700 * upd + 96 * memb

@@ -81,12 +81,13 @@ def test_fp64_fwd64_matches_oracle_f64(order):
     assert max(errs.values()) <= 1e-10, errs
 
 
-@pytest.mark.parametrize("precision,bar", [("fp16c", 1e-2), ("fp16t", 1e-2), ("fp16", 2.5e-2)])
+@pytest.mark.parametrize("precision,bar", [("fp16c", 1e-2), ("fp16t", 1e-2), ("fp16m", 1e-2), ("fp16", 2.5e-2)])
 def test_half_precision_fwd64_vs_fp64_oracle(precision, bar):
     """Receiver traces of the 16-bit storage modes against the FP64 oracle, 500 steps.
-    fp16c (binary16 + 8-bit residual on v, tau, p) and fp16t (residual on tau, p) must meet
-    1e-2; plain binary16 storage (every array binary16, as the reference's `half` arrays) is
-    held to its measured level."""
+    fp16c (binary16 + 8-bit residual on v, tau, p), fp16t (residual on tau, p) and fp16m
+    (24-bit tau, p; binary16 v, q and moduli) must meet the 1e-2 target; plain binary16
+    storage (every array binary16, as the reference's `half` arrays) misses it: 2.5e-2 is
+    its measured level (1.6e-2) with headroom, not the target."""
     print(f"fwd64 {precision} order 4", flush=True)
     cfg = configs.fwd64(order=4)
     nt = cfg["nt"]
@@ -99,11 +100,13 @@ def test_half_precision_fwd64_vs_fp64_oracle(precision, bar):
     assert max(errs) <= bar
 
 
-@pytest.mark.parametrize("precision,bar", [("fp16c", 1e-2), ("fp16t", 1e-2), ("fp16", 5e-2)])
+@pytest.mark.parametrize("precision,bar", [("fp16c", 1e-2), ("fp16t", 1e-2), ("fp16m", 1e-2), ("fp16", 5e-2)])
 def test_half_precision_perturbed_medium_vs_gpu_fp64(precision, bar):
     """Layered medium with +-10 % Gaussian-correlated perturbations of Kb, mu, phi (the
     512 x 512 x 384 workload's medium statistics) on a 192 x 192 x 160 grid, 1200 steps,
-    64 x 64 surface receivers: 16-bit modes against the GPU FP64 path."""
+    64 x 64 surface receivers: 16-bit modes against the GPU FP64 path.  The target 1e-2 holds
+    for fp16c, fp16t and fp16m; plain fp16's 5e-2 is its measured level (1.6e-2) with
+    headroom, not the target."""
     print(f"perturbed proxy {precision}", flush=True)
     cfg = configs.perturbed512(nt=1200, nx=192, ny=192, nz=160, stride=3)
     ref_s = configs.make_solver(cfg, precision="fp64")
@@ -125,7 +128,8 @@ def test_half_precision_full_size_perturbed512():
     sets in one run: the 128 x 128 surface receivers (vx, vz, p) and the reference's own layout,
     the vertical line through the source column (kernel.cu:2027-2031; the source cell left
     out) recording txx.  FP16C (binary16 + 8-bit residual on v, tau, p) and FP16T (residual on
-    tau, p) within 1e-2 of the GPU FP64 path on both; plain FP16 is reported."""
+    tau, p) and FP16M (24-bit tau, p; binary16 v, q and moduli) within 1e-2 of the GPU FP64
+    path on both; plain FP16 is reported (it misses 1e-2: 2.0e-2 measured)."""
     import time
 
     from biot_amd._lib import FIELD_INDEX
@@ -142,7 +146,7 @@ def test_half_precision_full_size_perturbed512():
     ci = {f: order.index(FIELD_INDEX[f]) for f in fields}
     print(f"  medium built in {time.time() - t0:.0f} s", flush=True)
     traces = {}
-    for prec in ("fp64", "fp16c", "fp16t", "fp16"):
+    for prec in ("fp64", "fp16c", "fp16t", "fp16m", "fp16"):
         t0 = time.time()
         s = configs.make_solver(cfg, precision=prec)
         s.run(cfg["nt"])
@@ -152,15 +156,15 @@ def test_half_precision_full_size_perturbed512():
         print(f"  {prec}: {ms / 1e3:.1f} s loop, {time.time() - t0:.0f} s total", flush=True)
     ref = traces["fp64"]
     surf = {p: [rel_l2(traces[p][ci[f], :, :nsurf], ref[ci[f], :, :nsurf]) for f in ("vx", "vz", "p")]
-            for p in ("fp16c", "fp16t", "fp16")}
+            for p in ("fp16c", "fp16t", "fp16m", "fp16")}
     cols = {p: {f: rel_l2(traces[p][ci[f], :, nsurf:], ref[ci[f], :, nsurf:]) for f in fields} for p in surf}
     col = {p: cols[p]["txx"] for p in surf}
     for p in surf:
         print(f"  {p} surface traces rel L2 vs fp64 (vx, vz, p) = " + " ".join(f"{x:.3e}" for x in surf[p]) +
               "; source column (txx, vx, vz, p) = " +
               " ".join(f"{cols[p][f]:.3e}" for f in ("txx", "vx", "vz", "p")))
-    assert all(np.isfinite(traces[p]).all() for p in ("fp16c", "fp16t", "fp16"))
-    for p in ("fp16c", "fp16t"):
+    assert all(np.isfinite(traces[p]).all() for p in ("fp16c", "fp16t", "fp16m", "fp16"))
+    for p in ("fp16c", "fp16t", "fp16m"):
         assert max(surf[p]) <= 1e-2 and col[p] <= 1e-2
 
 
@@ -200,7 +204,7 @@ def test_full_size_random_state_matches_oracle(precision, mode, tol):
     s.close()
 
 
-@pytest.mark.parametrize("precision,tol", [("fp16", 3e-2), ("fp16c", 1e-3), ("fp16t", 1e-2)])
+@pytest.mark.parametrize("precision,tol", [("fp16", 3e-2), ("fp16c", 1e-3), ("fp16t", 1e-2), ("fp16m", 1e-2)])
 def test_storage_scale_change_mid_run(precision, tol):
     """A new source amplitude changes the 16-bit storage scales, and the handle re-expresses
     every stored field and C-PML carry in the new units.  A source that is silent after step
@@ -299,3 +303,100 @@ def test_snapshot_reset_determinism_and_errors():
         t.run(1)
     assert e.value.status == 3
     t.close()
+
+
+def _fluid_column_medium(shape):
+    """Layered rock with a fluid-filled column (mu = Kb = 0: a fluid cell, C1 = 0) through the
+    grid along z, next to the source."""
+    m = media.layered(shape)
+    nz, ny, nx = shape
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    hole = ((ii - nx // 2 - 6) ** 2 + (jj - ny // 2) ** 2) <= 4
+    for k in ("mu", "Kb"):
+        m[k][:, hole] = 0.0
+    return m
+
+
+@pytest.mark.parametrize("setup", ["medium", "medium_device", "params"])
+def test_fp16m_coefficients_from_moduli(setup):
+    """FP16M forms the staggered averages and all 22 coefficients in the kernels from the
+    stored cell moduli.  On a layered medium with a fluid column (harmonic shear means with a
+    zero mu, the C1 = 0 guard of the fluid-velocity coefficients) and a ragged 72 x 61 x 66
+    grid, 400 steps, its receiver traces stay within the same 1e-2 of the GPU FP64 path as
+    FP16T, which reads the reference's coefficient arrays (FP16T is measured beside it); the
+    three set-up paths give the same storage."""
+    print(f"fp16m coefficients from moduli via {setup}", flush=True)
+    nx, ny, nz, nt = 72, 61, 66, 400
+    shape = (nz, ny, nx)
+    m = _fluid_column_medium(shape)
+    src = (nx // 2, ny // 2, nz // 2)
+    rec = [(i, j, 12) for i in range(8, nx - 8, 4) for j in range(8, ny - 8, 4)]
+    rec += [(nx // 2 + 6, ny // 2, k) for k in range(10, nz - 10, 3)]  # inside the fluid column
+    cfg = dict(nx=nx, ny=ny, nz=nz, dx=5.0, dt=4e-4, order=4, npml=10, f0=25.0, src=src,
+               src_fields=configs.SRC_EXPLOSIVE, medium=m, rec=rec, rec_fields=configs.REC_FIELDS)
+    out = {}
+    for prec in ("fp64", "fp16t", "fp16m"):
+        s = configs.make_solver(cfg, precision=prec, nt=nt)
+        if prec == "fp16m" and setup == "medium":
+            s.set_medium(m)
+        elif prec == "fp16m" and setup == "params":
+            s.set_params(media.params_from_medium(m))
+        s.run(nt)
+        out[prec] = s.traces()
+        s.close()
+    errs = {p: [rel_l2(out[p][c], out["fp64"][c]) for c in range(3)] for p in ("fp16t", "fp16m")}
+    for p, e in errs.items():
+        print(f"  {p} vs fp64 (vx, vz, p): " + " ".join(f"{x:.3e}" for x in e))
+    assert np.isfinite(out["fp16m"]).all() and np.abs(out["fp64"]).max() > 0
+    assert max(errs["fp16m"]) <= 1e-2
+
+
+def test_fp16m_refusals():
+    """FP16M needs the moduli and has no adjoint or stored coefficient arrays."""
+    s = Solver(24, 24, 24, 5.0, 4e-4, order=4, precision="fp16m", npml=4)
+    with pytest.raises(BiotError) as e:
+        s.set_coefficients([np.ones(24 ** 3, np.float32)] * 23)
+    assert e.value.status == 3
+    s.set_medium(media.homogeneous((24, 24, 24)))
+    with pytest.raises(BiotError) as e:
+        s.kernel_coefficient(0)
+    assert e.value.status == 3
+    s.set_receivers([100], ("p",), 8)
+    s.set_source(s.cell(12, 12, 12), np.ones(8, np.float32), ("p",))
+    with pytest.raises(BiotError):
+        s.forward_checkpointed(8, 4)
+    s.run(8)
+    assert np.isfinite(s.traces()).all()
+    s.close()
+
+
+def _w24_numpy(x32):
+    """24-bit float of a float32 array: round to nearest even on the low byte, keep 3 bytes."""
+    b = x32.view(np.uint32).astype(np.uint64)
+    b = (b + 0x7F + ((b >> 8) & 1)) & 0xFFFFFF00
+    return b.astype(np.uint32).view(np.float32)
+
+
+def test_fp16m_storage_round_trip():
+    """A written stress field reads back as its 24-bit float (upper three bytes of the float of
+    value / scale, rounded to nearest even) times the scale; the velocity as binary16."""
+    n = 24
+    s = Solver(n, n, n, 5.0, 4e-4, order=4, precision="fp16m", npml=4)
+    s.set_source(s.cell(12, 12, 12), np.ones(8, np.float32), ("p",))
+    s.set_medium(media.homogeneous((n, n, n)))
+    sc = s.field_scales()
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(n ** 3) * 10.0 ** rng.uniform(-6, 3, n ** 3) * sc["txx"]
+    # floats whose dropped byte is exactly 0x80 (ties), both parities of the kept bits
+    ties = (np.uint32(0x3F800000) + (np.arange(64, dtype=np.uint32) << 8) + np.uint32(0x80)).view(np.float32)
+    v[:64] = ties.astype(np.float64) * sc["txx"]
+    s.write_field("txx", v.reshape(n, n, n))
+    got = s.field("txx", f64=True).reshape(-1)
+    x32 = (v * (1.0 / sc["txx"])).astype(np.float32)
+    want = _w24_numpy(x32).astype(np.float64) * sc["txx"]
+    assert np.array_equal(got, want)
+    s.write_field("vx", (v / sc["txx"] * sc["vx"] * 1e-3).reshape(n, n, n))
+    gv = s.field("vx", f64=True).reshape(-1)
+    wv = (v / sc["txx"] * sc["vx"] * 1e-3 * (1.0 / sc["vx"])).astype(np.float32).astype(np.float16).astype(np.float64) * sc["vx"]
+    assert np.array_equal(gv, wv)
+    s.close()

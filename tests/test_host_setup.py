@@ -91,14 +91,14 @@ def test_argument_errors_are_status_codes():
     assert L.biot_create(C.byref(bad), C.byref(h)) == 1
     bad.nx, bad.npml = 64, 40
     assert L.biot_create(C.byref(bad), C.byref(h)) == 1
-    for prec in (-1, 5, 7):  # BIOT_FP32 .. BIOT_FP16T are 0..4
+    for prec in (-1, 6, 7):  # BIOT_FP32 .. BIOT_FP16M are 0..5
         bad.npml, bad.precision = 10, prec
         assert L.biot_create(C.byref(bad), C.byref(h)) == 1
     assert L.biot_ricker(C.c_float(-1.0), C.c_float(1e-3), 4, None) == 1
     assert L.biot_status_string(4) == b"non-finite value detected"
 
 
-@pytest.mark.parametrize("precision", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("precision", [0, 1, 2, 3, 4, 5])
 def test_create_without_device_fails_loudly(precision):
     """Without a GPU the library reports a HIP error instead of falling back to the CPU (every
     storage mode passes the argument checks and reaches the device)."""
