@@ -36,17 +36,19 @@ def slab_counts(n, L, R, h):
 
 
 # element bytes per storage mode: field, coefficient, C-PML carry, residual (FP16C on the 10
-# compensated fields v, tau, p; FP16T on the 7 of tau, p; FP16M's third byte of the 24-bit
+# compensated fields v, tau, p; FP16T on the 7 of tau, p; FP16M's third byte of its 24-bit
 # tau, p); FP16 keeps every array in binary16, FP16C, FP16T and FP16M keep float carries
 WIDTHS = {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 2, 0), "fp16c": (2, 2, 4, 1), "fp16t": (2, 2, 4, 1),
           "fp16m": (2, 2, 4, 1), "fp64": (8, 8, 8, 0)}
-# residual accesses per updated cell (velocity kernel, stress kernel, one fused step)
-RESIDUAL_ACCESSES = {"fp16c": (13, 17, 20), "fp16t": (7, 14, 14), "fp16m": (7, 14, 14)}
+# residual bytes per updated cell (velocity kernel, stress kernel, one fused step): FP16C and
+# FP16T one per access of a compensated field; FP16M the cell's 8-byte low-byte record of its
+# 24-bit tau, p (read by the velocity kernel, read and written by the stress kernel)
+RESIDUAL_ACCESSES = {"fp16c": (13, 17, 20), "fp16t": (7, 14, 14), "fp16m": (8, 16, 16)}
 # medium bytes per updated cell where they are not the coefficient arrays: FP16M reads the
-# cell's moduli records (velocity: rho, rho_f, C1, C2 in 8 bytes; stress: C, M, lambda_c, H in
-# 8 bytes and 1/mu in 2; the neighbours' records the averages use are re-reads of the same
-# arrays)
-MEDIUM_BYTES = {"fp16m": (8, 10, 18)}
+# cell's moduli records (velocity: rho, rho_f, C1, C2 in 8 bytes; stress: C, M, lambda_c, H and
+# the three shear means in 16 bytes; the neighbours' velocity records the averages use are
+# re-reads of the same array)
+MEDIUM_BYTES = {"fp16m": (8, 16, 24)}
 
 
 def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):
@@ -58,7 +60,8 @@ def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):
     slab.  The fused-step figure is 26 field + 22 coefficient accesses per cell plus
     16 carry accesses per slab membership (one read and one write of 8 carries); FP16C adds
     one residual byte per access of v, tau, p (V: 10 reads + 3 writes, T: 10 + 7, step 20),
-    FP16T per access of tau, p (V: 7 reads, T: 7 + 7, step 14)."""
+    FP16T per access of tau, p (V: 7 reads, T: 7 + 7, step 14); FP16M reads its 8-byte
+    low-byte record in V and reads and writes it in T (step 16)."""
     wf, wk, wc, wr = WIDTHS[mode]
     rv, rt, rstep = RESIDUAL_ACCESSES.get(mode, (0, 0, 0))
     kv, kt, kstep = MEDIUM_BYTES.get(mode, (14 * wk, 8 * wk, 22 * wk))
@@ -303,6 +306,14 @@ def batch_shots(nt=1000, every=50, graph_steps=50):
     obs = [b.traces(i) for i in range(len(cfg["shots"]))]
     b.forward(nt, graph_steps=graph_steps)
     f_ms = b.last_ms()
+    # the same tile shape on persistent grids (blocks striding through the tile list) beside
+    # the tuned launch: the measurement behind the autotuner's choice
+    persistent = {}
+    for blocks in (1024, 2048, 4096):
+        b.set_launch(launch[0], launch[1], launch[2], blocks)
+        b.forward(nt, graph_steps=graph_steps)
+        persistent[str(blocks)] = round(cells * nt / b.last_ms() / 1e3, 1)
+    b.set_launch(*launch)
     # observed data: the same shots with a slower, softer start model would need a second
     # model; the gradient's cost does not depend on the data, so the recorded traces scaled
     # by 1.01 stand in for observations
@@ -318,6 +329,9 @@ def batch_shots(nt=1000, every=50, graph_steps=50):
                 shots=len(cfg["shots"]), cells=cells, nt=nt,
                 forward_ms=round(f_ms, 1), forward_Mcell_updates_per_s=round(cells * nt / f_ms / 1e3, 1),
                 graph_steps=graph_steps, launch_bx_by_zchunk_blocks=list(launch),
+                grid_kind=(f"persistent ({launch[3]} blocks)" if launch[3] in (1024, 2048, 4096)
+                           else f"one block per tile ({launch[3]} blocks)"),
+                persistent_grid_Mcell_updates_per_s=persistent, policy="fp16 (binary16 wavefields)",
                 gradient_ms=round(g_ms, 1), gradient_ms_per_shot=round(g_ms / len(cfg["shots"]), 1),
                 checkpoint_every=every, gradient_groups=groups, misfit=J, finite=finite)
 

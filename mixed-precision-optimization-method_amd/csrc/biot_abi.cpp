@@ -52,30 +52,34 @@ double half_res_unit(_Float16 h)
   return std::ldexp(1.0, e - 33);
 }
 
-// FP16M's 24-bit float: the upper two bytes of the float in a 16-bit array, the third byte in
-// a byte array, rounded to nearest even on the dropped byte
-void w24_store(float x, uint16_t &hi, int8_t &lo)
+// FP16M's 24-bit float: the upper three bytes of the float, rounded to nearest even on the
+// dropped byte; the upper two in the field array, the third in the cell's low-byte record
+uint32_t w24_bits(float x)
 {
-  uint32_t b;
-  std::memcpy(&b, &x, 4);
-  b = b + 0x7Fu + ((b >> 8) & 1u);
-  hi = (uint16_t)(b >> 16);
-  lo = (int8_t)(uint8_t)(b >> 8);
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  return u + 0x7Fu + ((u >> 8) & 1u);
 }
-double w24_value(uint16_t hi, int8_t lo)
+double w24_value(uint16_t hi, uint8_t lo)
 {
-  const uint32_t b = ((uint32_t)hi << 16) | ((uint32_t)(uint8_t)lo << 8);
+  const uint32_t u = ((uint32_t)hi << 16) | ((uint32_t)lo << 8);
   float f;
-  std::memcpy(&f, &b, 4);
+  std::memcpy(&f, &u, 4);
   return (double)f;
 }
 
-// host storage-type conversion; lo: the residuals of a compensated field, else null
-void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst, int8_t *lo)
+// host storage-type conversion; lo: the residuals of a compensated field (stride 1) or the
+// low bytes of an FP16M 24-bit field (w24, stride lo_stride), else null
+void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst, int8_t *lo,
+                bool w24 = false, size_t lo_stride = 1)
 {
-  if (prec == BIOT_FP16M && lo) {
+  if (w24) {
     uint16_t *d = (uint16_t *)dst;
-    for (int64_t c = 0; c < n; ++c) w24_store((float)(src[c] * scale_inv), d[c], lo[c]);
+    for (int64_t c = 0; c < n; ++c) {
+      const uint32_t r = w24_bits((float)(src[c] * scale_inv));
+      d[c] = (uint16_t)(r >> 16);
+      lo[c * lo_stride] = (int8_t)(uint8_t)(r >> 8);
+    }
     return;
   }
   if (prec == BIOT_FP32) {
@@ -96,11 +100,12 @@ void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *
     for (int64_t c = 0; c < n; ++c) d[c] = src[c] * scale_inv;
   }
 }
-void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double scale, double *dst)
+void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double scale, double *dst,
+                  bool w24 = false, size_t lo_stride = 1)
 {
-  if (prec == BIOT_FP16M && lo) {
+  if (w24) {
     const uint16_t *s = (const uint16_t *)src;
-    for (int64_t c = 0; c < n; ++c) dst[c] = w24_value(s[c], lo[c]) * scale;
+    for (int64_t c = 0; c < n; ++c) dst[c] = w24_value(s[c], (uint8_t)lo[c * lo_stride]) * scale;
     return;
   }
   if (prec == BIOT_FP32) {
@@ -118,6 +123,9 @@ void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double
     for (int64_t c = 0; c < n; ++c) dst[c] = s[c] * scale;
   }
 }
+
+// bytes spanned by the residual / low bytes of n cells of field f from lo_ptr(f)
+size_t lo_span(const biot_handle *h, int f, int64_t n) { return (size_t)(n - 1) * h->lo_stride(f) + 1; }
 
 }  // namespace
 
@@ -233,8 +241,8 @@ extern "C" int biot_create(const biot_grid *g, biot_handle **out)
   const int64_t lim = (int64_t)1 << 32;
   if (h->n * (int64_t)h->esz >= lim)
     return refuse(h, BIOT_ERR_ARG, "grid too large: one field array would exceed 4 GiB");
-  if (h->prec == BIOT_FP16M && h->n * 8 >= lim)
-    return refuse(h, BIOT_ERR_ARG, "grid too large: an FP16M medium record array would exceed 4 GiB");
+  if (h->prec == BIOT_FP16M && h->n * 16 >= lim)
+    return refuse(h, BIOT_ERR_ARG, "grid too large: the FP16M stress medium records (16 bytes a cell) would exceed 4 GiB");
   for (int ax = 0; ax < 3; ++ax)
     if (NCARRY * h->slab[ax] * (int64_t)h->csz >= lim)
       return refuse(h, BIOT_ERR_ARG, "absorbing slabs too large: one axis's C-PML memory would exceed 4 GiB");
@@ -555,8 +563,8 @@ extern "C" int biot_set_receivers(biot_handle *h, const int64_t *cells, int64_t 
 extern "C" int biot_reset(biot_handle *h)
 {
   if (!h) return BIOT_ERR_ARG;
-  HIPCHK(h, hipMemsetAsync(h->d_fields, 0, (size_t)NFIELD * h->n * h->esz, h->stream));
-  if (h->d_lo) HIPCHK(h, hipMemsetAsync(h->d_lo, 0, (size_t)NFIELD * h->n, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d_fields, 0, h->fields_bytes(), h->stream));
+  if (h->d_lo) HIPCHK(h, hipMemsetAsync(h->d_lo, 0, h->lo_bytes(), h->stream));
   for (int ax = 0; ax < 3; ++ax)
     if (h->slab[ax] > 0)
       HIPCHK(h, hipMemsetAsync(h->d_carry[ax], 0, (size_t)NCARRY * h->slab[ax] * h->csz, h->stream));
@@ -848,10 +856,11 @@ extern "C" int biot_read_field_f64(biot_handle *h, int32_t field, double *out)
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(tmp.data(), h->field_ptr(field), tmp.size(), hipMemcpyDeviceToHost));
   if (h->lo_ptr(field)) {
-    lo.resize((size_t)h->n);
+    lo.resize(lo_span(h, field, h->n));
     HIPCHK(h, hipMemcpy(lo.data(), h->lo_ptr(field), lo.size(), hipMemcpyDeviceToHost));
   }
-  from_storage(h->prec, tmp.data(), lo.empty() ? nullptr : lo.data(), h->n, h->fscale[field], out);
+  from_storage(h->prec, tmp.data(), lo.empty() ? nullptr : lo.data(), h->n, h->fscale[field], out, h->w24(field),
+               h->lo_stride(field));
   return BIOT_OK;
 }
 
@@ -870,9 +879,12 @@ extern "C" int biot_write_field_f64(biot_handle *h, int32_t field, const double 
   if (!h || !in) return BIOT_ERR_ARG;
   if (field < 0 || field >= NFIELD) return fail(h, BIOT_ERR_ARG, "field %d out of range", field);
   std::vector<unsigned char> tmp((size_t)h->n * h->esz);
-  std::vector<int8_t> lo(h->lo_ptr(field) ? (size_t)h->n : 0);
-  to_storage(h->prec, in, h->n, 1.0 / h->fscale[field], tmp.data(), lo.empty() ? nullptr : lo.data());
+  std::vector<int8_t> lo(h->lo_ptr(field) ? lo_span(h, field, h->n) : 0);
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  // the low bytes share their records with the other 24-bit fields: read, patch, write back
+  if (h->w24(field)) HIPCHK(h, hipMemcpy(lo.data(), h->lo_ptr(field), lo.size(), hipMemcpyDeviceToHost));
+  to_storage(h->prec, in, h->n, 1.0 / h->fscale[field], tmp.data(), lo.empty() ? nullptr : lo.data(),
+             h->w24(field), h->lo_stride(field));
   HIPCHK(h, hipMemcpy(h->field_ptr(field), tmp.data(), tmp.size(), hipMemcpyHostToDevice));
   if (!lo.empty())
     HIPCHK(h, hipMemcpy(h->lo_ptr(field), lo.data(), lo.size(), hipMemcpyHostToDevice));
@@ -885,22 +897,23 @@ extern "C" int biot_snapshot_xz(biot_handle *h, int32_t field, int32_t j, float 
   if (field < 0 || field >= NFIELD) return fail(h, BIOT_ERR_ARG, "field %d out of range", field);
   if (j < 0 || j >= h->g.ny) return fail(h, BIOT_ERR_ARG, "y index %d outside grid", j);
   const int nx = h->g.nx, nz = h->g.nz;
-  std::vector<unsigned char> row((size_t)nx * nz * h->esz);
+  const size_t eb = h->esz;
+  std::vector<unsigned char> row((size_t)nx * nz * eb);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   // one 2-D copy: nz rows of nx elements, source pitch nx*ny elements
-  HIPCHK(h, hipMemcpy2D(row.data(), (size_t)nx * h->esz,
-                        (char *)h->field_ptr(field) + (size_t)j * nx * h->esz,
-                        (size_t)nx * h->g.ny * h->esz, (size_t)nx * h->esz, (size_t)nz,
-                        hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy2D(row.data(), (size_t)nx * eb, (char *)h->field_ptr(field) + (size_t)j * nx * eb,
+                        (size_t)nx * h->g.ny * eb, (size_t)nx * eb, (size_t)nz, hipMemcpyDeviceToHost));
   std::vector<int8_t> lo;
+  const size_t ls = h->lo_stride(field);
   if (h->lo_ptr(field)) {
-    lo.resize((size_t)nx * nz);
-    HIPCHK(h, hipMemcpy2D(lo.data(), (size_t)nx, h->lo_ptr(field) + (size_t)j * nx,
-                          (size_t)nx * h->g.ny, (size_t)nx, (size_t)nz, hipMemcpyDeviceToHost));
+    // nz rows of nx cells at stride ls (the last cell's byte ends each row)
+    lo.resize((size_t)nx * nz * ls);
+    HIPCHK(h, hipMemcpy2D(lo.data(), (size_t)nx * ls, h->lo_ptr(field) + (size_t)j * nx * ls,
+                          (size_t)nx * h->g.ny * ls, lo_span(h, field, nx), (size_t)nz, hipMemcpyDeviceToHost));
   }
   std::vector<double> tmp((size_t)nx * nz);
   from_storage(h->prec, row.data(), lo.empty() ? nullptr : lo.data(), (int64_t)nx * nz,
-               h->fscale[field], tmp.data());
+               h->fscale[field], tmp.data(), h->w24(field), ls);
   for (size_t c = 0; c < tmp.size(); ++c) out[c] = (float)tmp[c];
   return BIOT_OK;
 }

@@ -28,11 +28,11 @@
 //         a third of FP16C's residual traffic.
 //   FP16M v and q binary16 (scaled); tau and p 24-bit floats, the upper three bytes of a
 //         float (sign, 8 exponent bits, 15 stored significand bits, rounded to nearest even),
-//         kept as a 16-bit array of the upper two bytes and the int8 array of the third; the
-//         medium as three per-cell records of binary16 moduli, scaled to [-1, 1] (mv: rho,
-//         rho_f, C1, C2; mt: C, M, lambda_c, H; mr: 1/mu, +inf where mu = 0), from which the
-//         kernels form the staggered averages and the 22 coefficients in registers; float
-//         C-PML carries.
+//         packed 3 bytes per cell (one unaligned 4-byte load per value); the medium as two
+//         per-cell records of binary16 moduli scaled to [-1, 1] (mv:
+//         rho, rho_f, C1, C2 in 8 bytes; mt: C, M, lambda_c, H and the three 4-cell harmonic
+//         means of mu in 16 bytes), from which the kernels form the staggered averages and
+//         the 22 coefficients in registers; float C-PML carries.
 //   FP64  everything double.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -49,6 +49,11 @@ constexpr uint32_t comp_fields(int prec)
 {
   return prec == PREC_FP16C ? COMP_FIELDS_C : (prec == PREC_FP16T || prec == PREC_FP16M) ? COMP_FIELDS_T : 0u;
 }
+// FP16M: the fields stored as 24-bit floats (txx .. tyz, p): the upper 16 bits in the field's
+// own array, the third byte in the cell's 8-byte low-byte record, at byte f - W24_FIRST (the
+// record's last byte is unused)
+constexpr uint32_t W24_FIELDS = COMP_FIELDS_T;
+constexpr int W24_FIRST = 6, W24_RECORD = 8;
 // kernel coefficient order
 enum {
   K_U1X = 0, K_U1Y, K_U1Z, K_U2X, K_U2Y, K_U2Z, K_W1X, K_W1Y, K_W1Z, K_W2X, K_W2Y, K_W2Z,
@@ -61,8 +66,8 @@ enum {
 // h * (stored(c) + stored(c + axis)); then U1 = cU1 / rho, U2 = kU2 rho_f / rho,
 // m = C2 - rho_f^2 / rho, a = aH C1, W1 = (m - a) / (m + a), W2 = cW2 (rho_f / rho) / (m + a),
 // W3 = cW3 / (m + a) (main.cpp:447-481); P1 = kP1 C, P2 = kP2 M, S1 = kS1 lambda_c, S2 = kS2 H,
-// S3 = kS3 C, Sij = kSh / (sum of the stored 1/mu over the 4-cell quad) (main.cpp:326-337,
-// 483-494), all on the stored values.
+// S3 = kS3 C, Sij = kSh mu_ij (the stored 4-cell harmonic mean, main.cpp:326-337, 483-494), all
+// on the stored values.
 enum {
   MC_HRHO = 0, MC_HRHOF, MC_HC1, MC_HC2, MC_KU2, MC_CU1, MC_CW2, MC_CW3, MC_AH,
   MC_KP1, MC_KP2, MC_KS1, MC_KS2, MC_KS3, MC_KSH, NMC
@@ -78,8 +83,8 @@ struct StepArgs {
   // in 64 bits: these arrays may span more than 4 GiB)
   const void *kv, *kt;
   float ks[NKC];             // per-array scale (binary16 storage), 1 otherwise
-  // FP16M: the medium records (mv, mt: 4 binary16 values per cell; mr: one) and constants
-  const void *mv, *mt, *mr;
+  // FP16M: the medium records (mv: 4 binary16 values per cell, mt: 8) and constants
+  const void *mv, *mt;
   float mc[NMC];
   // per axis: the C-PML carries (carry type) of the slab cells, in two groups of records — the
   // velocity half step's carries 0..3, then the stress half step's 4..7 — each group slab[ax]
@@ -182,6 +187,8 @@ hipError_t launch_set_counter(int *it, int v, hipStream_t s);
 // multiply a stored array by f (lo: its residuals or null); carries use prec of the carry
 // type, i.e. pass PREC_FP32 for FP16C/FP16T carries and PREC_FP16 for FP16 carries
 hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
+// the same for an FP16M 24-bit array of n values
+hipError_t launch_rescale24(void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
 // element e of every 4-element record times f[e] (carry groups whose carries scale differently)
 hipError_t launch_rescale4(int prec, void *p, int64_t nrec, const double f[4], hipStream_t s);
 // the xz slice y = j of one field as float values (stored value times scale): out[k*nx + i]

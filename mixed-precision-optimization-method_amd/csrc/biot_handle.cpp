@@ -241,11 +241,20 @@ biot_handle::~biot_handle()
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+size_t biot_handle::fields_bytes() const { return (size_t)NFIELD * n * esz; }
+
 void *biot_handle::field_ptr(int f) const { return (char *)d_fields + (size_t)f * n * esz; }
+
+size_t biot_handle::lo_bytes() const
+{
+  if (!comp_fields(prec)) return 0;
+  return prec == BIOT_FP16M ? (size_t)W24_RECORD * n : (size_t)NFIELD * n;
+}
 
 int8_t *biot_handle::lo_ptr(int f) const
 {
   if (!d_lo || !((comp_fields(prec) >> f) & 1u)) return nullptr;
+  if (w24(f)) return d_lo + (f - W24_FIRST);  // the field's byte of record 0
   return d_lo + (size_t)f * n;
 }
 
@@ -265,8 +274,7 @@ void biot_handle::fill_args(StepArgs &a) const
   if (prec == BIOT_FP16M) {
     for (int k = 0; k < NKC; ++k) a.k[k] = nullptr;
     a.mv = d_coef;
-    a.mt = (const char *)d_coef + (size_t)8 * n;
-    a.mr = (const char *)d_coef + (size_t)16 * n;
+    a.mt = (const char *)d_coef + mt_offset();
     for (int q = 0; q < NMC; ++q) a.mc[q] = mc[q];
   }
   for (int ax = 0; ax < 3; ++ax) {
@@ -301,12 +309,12 @@ int biot_handle::allocate()
     device_bytes += (int64_t)bytes;
     return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
   };
-  // FP16M keeps the medium records (18 bytes per cell) in place of the coefficient arrays
-  const size_t coef_bytes = prec == BIOT_FP16M ? (size_t)18 * n : (size_t)NKC * n * esz;
+  // FP16M keeps the medium records (8 + 16 bytes per cell) in place of the coefficient arrays
+  const size_t coef_bytes = prec == BIOT_FP16M ? mt_offset() + (size_t)16 * n : (size_t)NKC * n * esz;
   const size_t pack_bytes = prec == BIOT_FP16M ? 0 : (size_t)NKC * n * esz;
-  bool ok = alloc(&d_fields, (size_t)NFIELD * n * esz) && alloc(&d_coef, coef_bytes) &&
+  bool ok = alloc(&d_fields, fields_bytes()) && alloc(&d_coef, coef_bytes) &&
             alloc(&d_kpack, pack_bytes) && alloc((void **)&d_flag, 256) && clear_nonfinite() == BIOT_OK;
-  if (ok && comp_fields(prec)) ok = alloc((void **)&d_lo, (size_t)NFIELD * n);
+  if (ok) ok = alloc((void **)&d_lo, lo_bytes());
   for (int ax = 0; ax < 3 && ok; ++ax) ok = alloc(&d_carry[ax], (size_t)NCARRY * slab[ax] * csz);
   const int dims[3] = {g.nx, g.ny, g.nz};
   const size_t asz = prec == BIOT_FP64 ? 8 : 4;
@@ -495,8 +503,11 @@ int biot_handle::rescale_state(const float old_scale[NFIELD])
     return f == 1.0 || launch_rescale(p_, p, lo, cnt, f, stream) == hipSuccess;
   };
   bool ok = true;
-  for (int f = 0; f < NFIELD && ok; ++f)
-    ok = rescale(prec, field_ptr(f), lo_ptr(f), n, (double)old_scale[f] / (double)fscale[f]);
+  for (int f = 0; f < NFIELD && ok; ++f) {
+    const double r = (double)old_scale[f] / (double)fscale[f];
+    if (w24(f)) ok = r == 1.0 || launch_rescale24(field_ptr(f), lo_ptr(f), n, r, stream) == hipSuccess;
+    else ok = rescale(prec, field_ptr(f), lo_ptr(f), n, r);
+  }
   const double rt = (double)old_scale[6] / fscale[6], rv = (double)old_scale[0] / fscale[0];
   const double rq = (double)old_scale[3] / fscale[3];
   for (int ax = 0; ax < 3 && ok; ++ax) {
@@ -653,17 +664,21 @@ int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
   return BIOT_OK;
 }
 
-// FP16M: the medium records from the 9 cell moduli.  Each stored quantity is scaled by its
-// largest magnitude (1/mu by its largest finite value; a zero mu stores +inf), rounded to
-// binary16; the kernels' constants fold the scales back together with the reference's
-// coefficient factors (Cv, Cp, DT, 1/dx as the kernels hold it, the 0.01 x 100 of the
-// stress update) and the field-storage ratios kfac.
+// FP16M: the medium records from the 9 cell moduli.  Velocity record per cell: rho, rho_f, C1,
+// C2; stress record: C, M, lambda_c, H and the three 4-cell harmonic means of mu of
+// main.cpp:326-337 (zero when a mu of the quad is zero, and on the last plane of each axis,
+// where the reference leaves them zero).  Each stored quantity is scaled by its largest
+// magnitude (the three means by one common scale) and rounded to binary16; the kernels'
+// constants fold the scales back together with the reference's coefficient factors (Cv, Cp,
+// DT, 1/dx as the kernels hold it, the 0.01 x 100 of the stress update) and the field-storage
+// ratios kfac.
 namespace {
 enum { MD_M = 0, MD_HH, MD_H2U, MD_C, MD_C1, MD_C2, MD_RHO, MD_RHOF, MD_MU };
 struct MedCtx {
   const std::vector<float> *mod;
-  double inv[9];       // 1 / scale per record slot: mv rho rhof C1 C2, mt C M H2u HH, mr 1/mu
-  uint16_t *mv, *mt, *mr;
+  int nx, ny, nz;
+  double sc[9];        // scales: rho rho_f C1 C2 | C M H2u HH | the shear means
+  uint16_t *mv, *mt;
   double mx[64][9];    // per-thread maxima (first pass)
   std::atomic<int> slot{0};
 };
@@ -672,17 +687,34 @@ inline uint16_t h16(double x)
   const _Float16 h = (_Float16)(float)x;
   return __builtin_bit_cast(uint16_t, h);
 }
-const int MED_SRC[9] = {MD_RHO, MD_RHOF, MD_C1, MD_C2, MD_C, MD_M, MD_H2U, MD_HH, MD_MU};
+const int MED_SRC[8] = {MD_RHO, MD_RHOF, MD_C1, MD_C2, MD_C, MD_M, MD_H2U, MD_HH};
+// the 4-cell harmonic means of mu at cell c (xy, xz, yz quads), main.cpp:326-337
+void shear_means(const MedCtx *cx, int64_t c, double out[3])
+{
+  const int nx = cx->nx, ny = cx->ny;
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
+  out[0] = out[1] = out[2] = 0.0;
+  if (i >= nx - 1 || j >= ny - 1 || k >= cx->nz - 1) return;
+  const float *mu = cx->mod[MD_MU].data();
+  auto hm = [&](int64_t a, int64_t b, int64_t d, int64_t e) {
+    if (mu[a] == 0 || mu[b] == 0 || mu[d] == 0 || mu[e] == 0) return 0.0;
+    return 1.0 / (0.25 * (1.0 / mu[a] + 1.0 / mu[b] + 1.0 / mu[d] + 1.0 / mu[e]));
+  };
+  out[0] = hm(c, c + 1, c + sy, c + 1 + sy);
+  out[1] = hm(c, c + 1, c + sz, c + 1 + sz);
+  out[2] = hm(c, c + sy, c + sz, c + sy + sz);
+}
 void med_max_body(int64_t lo, int64_t hi, void *p)
 {
   auto *cx = static_cast<MedCtx *>(p);
   double m[9] = {};
-  for (int64_t c = lo; c < hi; ++c)
-    for (int q = 0; q < 9; ++q) {
-      const double v = cx->mod[MED_SRC[q]][c];
-      const double a = q == 8 ? (v > 0 ? 1.0 / v : 0.0) : std::fabs(v);
-      m[q] = std::max(m[q], a);
-    }
+  double sm[3];
+  for (int64_t c = lo; c < hi; ++c) {
+    for (int q = 0; q < 8; ++q) m[q] = std::max(m[q], (double)std::fabs(cx->mod[MED_SRC[q]][c]));
+    shear_means(cx, c, sm);
+    m[8] = std::max({m[8], sm[0], sm[1], sm[2]});
+  }
   const int t = cx->slot.fetch_add(1);
   for (int q = 0; q < 9; ++q) cx->mx[t][q] = m[q];
 }
@@ -690,13 +722,15 @@ void med_pack_body(int64_t lo, int64_t hi, void *p)
 {
   auto *cx = static_cast<MedCtx *>(p);
   const std::vector<float> *M = cx->mod;
+  double sm[3];
   for (int64_t c = lo; c < hi; ++c) {
     for (int q = 0; q < 4; ++q) {
-      cx->mv[4 * c + q] = h16(M[MED_SRC[q]][c] * cx->inv[q]);
-      cx->mt[4 * c + q] = h16(M[MED_SRC[4 + q]][c] * cx->inv[4 + q]);
+      cx->mv[4 * c + q] = h16(M[MED_SRC[q]][c] / cx->sc[q]);
+      cx->mt[8 * c + q] = h16(M[MED_SRC[4 + q]][c] / cx->sc[4 + q]);
     }
-    const double mu = M[MD_MU][c];
-    cx->mr[c] = mu > 0 ? h16(1.0 / mu * cx->inv[8]) : h16(INFINITY);
+    shear_means(cx, c, sm);
+    for (int q = 0; q < 3; ++q) cx->mt[8 * c + 4 + q] = h16(sm[q] / cx->sc[8]);
+    cx->mt[8 * c + 7] = 0;
   }
 }
 }  // namespace
@@ -710,21 +744,21 @@ int biot_handle::upload_medium()
     }
   auto cx = std::make_unique<MedCtx>();
   cx->mod = mod;
+  cx->nx = g.nx, cx->ny = g.ny, cx->nz = g.nz;
   std::memset(cx->mx, 0, sizeof cx->mx);
   parallel_for(n, med_max_body, cx.get());
-  double sc[9] = {};
-  for (int t = 0; t < 64; ++t)
-    for (int q = 0; q < 9; ++q) sc[q] = std::max(sc[q], cx->mx[t][q]);
+  double *sc = cx->sc;
   for (int q = 0; q < 9; ++q) {
+    sc[q] = 0.0;
+    for (int t = 0; t < 64; ++t) sc[q] = std::max(sc[q], cx->mx[t][q]);
     if (!(sc[q] > 0) || !std::isfinite(sc[q])) sc[q] = 1.0;
-    cx->inv[q] = 1.0 / sc[q];
   }
-  std::vector<uint16_t> rec((size_t)9 * n);
-  cx->mv = rec.data();
-  cx->mt = rec.data() + (size_t)4 * n;
-  cx->mr = rec.data() + (size_t)8 * n;
+  std::vector<uint16_t> mv((size_t)4 * n), mt((size_t)8 * n);
+  cx->mv = mv.data();
+  cx->mt = mt.data();
   parallel_for(n, med_pack_body, cx.get());
-  if (hipMemcpy(d_coef, rec.data(), rec.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(d_coef, mv.data(), mv.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy((char *)d_coef + mt_offset(), mt.data(), mt.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
     err = "medium upload failed";
     return BIOT_ERR_HIP;
   }
@@ -744,6 +778,6 @@ int biot_handle::upload_medium()
   mc[MC_KS1] = (float)(sc[6] * str * kfac[K_S1]);
   mc[MC_KS2] = (float)(sc[7] * str * kfac[K_S2]);
   mc[MC_KS3] = (float)(sc[4] * str * kfac[K_S3]);
-  mc[MC_KSH] = (float)(4.0 * str * kfac[K_SXY] / sc[8]);
+  mc[MC_KSH] = (float)(sc[8] * str * kfac[K_SXY]);
   return BIOT_OK;
 }

@@ -21,8 +21,8 @@ struct biot_handle {
   std::string err;
 
   // device memory
-  void *d_fields = nullptr;          // 13 * n
-  int8_t *d_lo = nullptr;            // FP16C / FP16T residuals, 13 * n (compensated fields used)
+  void *d_fields = nullptr;          // 13 * n (FP16M: 6 binary16 arrays, then 7 packed 24-bit ones)
+  int8_t *d_lo = nullptr;  // FP16C / FP16T residuals, 13 * n (compensated fields used); FP16M low-byte records, 8 * n
   void *d_coef = nullptr;            // 22 * n (FP16M: the medium records, 18 bytes per cell)
   void *d_kpack = nullptr;           // the coefficients packed per cell (velocity, then stress block)
   void *d_carry[3] = {nullptr, nullptr, nullptr};
@@ -66,7 +66,16 @@ struct biot_handle {
 
   ~biot_handle();
   void *field_ptr(int f) const;
-  int8_t *lo_ptr(int f) const;  // residual array of field f, or null
+  // residual array of field f (FP16C / FP16T, stride 1), or FP16M's low byte of a 24-bit field
+  // in record 0 of the low-byte records (stride W24_RECORD), or null
+  int8_t *lo_ptr(int f) const;
+  size_t lo_stride(int f) const { return w24(f) ? biot::W24_RECORD : 1; }
+  size_t lo_bytes() const;
+  // FP16M keeps tau and p as 24-bit floats: the upper 16 bits in the field array, the low byte
+  // in a per-cell record shared by the seven fields
+  bool w24(int f) const { return prec == BIOT_FP16M && ((biot::W24_FIELDS >> f) & 1u); }
+  size_t fields_bytes() const;
+  size_t mt_offset() const { return ((size_t)8 * n + 255) & ~(size_t)255; }  // FP16M stress records
   bool half() const
   {
     return prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T || prec == BIOT_FP16M;

@@ -110,16 +110,50 @@ __device__ __forceinline__ int h_exp(_Float16 h)
 __device__ __forceinline__ float res_unit(int e) { return __builtin_bit_cast(float, (u32)(e + 94) << 23); }
 __device__ __forceinline__ float res_inv(int e) { return __builtin_bit_cast(float, (u32)(160 - e) << 23); }
 
-// 24-bit float: the upper two bytes of the float's bits in a 16-bit array, the third byte in
-// a byte array; rounded to nearest even on the dropped low byte (overflow rounds to inf)
-__device__ __forceinline__ float w24_load(uint16_t hi, uint8_t lo)
-{
-  return __builtin_bit_cast(float, ((u32)hi << 16) | ((u32)lo << 8));
-}
+// FP16M's 24-bit float: the upper three bytes of the float's bits, rounded to nearest even on
+// the dropped byte (overflow rounds to inf).  The upper 16 bits sit in the field's own array
+// (2 bytes per cell, like a binary16 field), the third byte in the cell's 8-byte low-byte
+// record shared by the seven 24-bit fields (byte f - W24_FIRST).  A stencil point of several
+// of those fields loads the record once (one 8-byte load), and the stress step writes it with
+// one 8-byte store; every access is aligned.
+typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ u32 w24_round(float x)
 {
   const u32 b = __builtin_bit_cast(u32, x);
   return b + 0x7Fu + ((b >> 8) & 1u);
+}
+// the float of upper half hi (zero-extended) and byte s (0..3) of record dword d
+__device__ __forceinline__ float w24_join(u32 hi, u32 d, int s)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_perm(hi, d, 0x0504000Cu | ((u32)s << 8)));
+}
+// value of 24-bit field F at cell idx: hi the field's array, rec the low-byte record array
+template <int F>
+__device__ __forceinline__ float w24_at(const void *hi, const void *rec, u32 idx)
+{
+  constexpr int s = F - W24_FIRST;
+  const dw2_t r = at<dw2_t>(rec, idx);
+  return w24_join((u32)at<uint16_t>(hi, idx), s < 4 ? r.x : r.y, s & 3);
+}
+// runtime field: lo points at the field's byte of record 0 (record base + f - W24_FIRST)
+__device__ __forceinline__ float w24_load(const void *hi, const int8_t *lo, u32 idx)
+{
+  const u32 b = reinterpret_cast<const uint8_t *>(lo)[idx * (u32)W24_RECORD];
+  return __builtin_bit_cast(float, ((u32)at<uint16_t>(hi, idx) << 16) | (b << 8));
+}
+__device__ __forceinline__ void w24_store(void *hi, int8_t *lo, u32 idx, float x)
+{
+  const u32 r = w24_round(x);
+  at<uint16_t>(hi, idx) = (uint16_t)(r >> 16);
+  reinterpret_cast<uint8_t *>(lo)[idx * (u32)W24_RECORD] = (uint8_t)(r >> 8);
+}
+// the low-byte record of seven rounded values (w24_round), r[s] for byte s
+__device__ __forceinline__ dw2_t w24_record(const u32 (&r)[7])
+{
+  dw2_t d;
+  d.x = ((r[0] >> 8) & 0xFFu) | (r[1] & 0xFF00u) | ((r[2] << 8) & 0xFF0000u) | ((r[3] << 16) & 0xFF000000u);
+  d.y = ((r[4] >> 8) & 0xFFu) | (r[5] & 0xFF00u) | ((r[6] << 8) & 0xFF0000u);
+  return d;
 }
 
 // value = stored part (+ residual for the compensated fields of the FP16C / FP16T policies;
@@ -130,7 +164,7 @@ __device__ __forceinline__ typename P::A load_raw(const void *f, const int8_t *l
 {
   using S = typename P::S;
   if constexpr (P::w24) {
-    if (comp_f) return w24_load(at<uint16_t>(f, idx), at<uint8_t>(lo, idx));
+    if (comp_f) return w24_load(f, lo, idx);
   }
   const S h = at<S>(f, idx);
   typename P::A v = (typename P::A)h;
@@ -145,7 +179,7 @@ __device__ __forceinline__ typename P::A load_own(const void *f, const int8_t *l
 {
   using S = typename P::S;
   if constexpr (P::w24) {
-    if (comp_f) return w24_load(at<uint16_t>(f, idx), at<uint8_t>(lo, idx));
+    if (comp_f) return w24_load(f, lo, idx);
   }
   S h;
   if constexpr (NT) h = __builtin_nontemporal_load(&at<S>(f, idx));
@@ -163,9 +197,7 @@ __device__ __forceinline__ void store_raw(void *f, int8_t *lo, bool comp_f, u32 
   using S = typename P::S;
   if constexpr (P::w24) {
     if (comp_f) {
-      const u32 r = w24_round(x);
-      at<uint16_t>(f, idx) = (uint16_t)(r >> 16);
-      at<uint8_t>(lo, idx) = (uint8_t)(r >> 8);
+      w24_store(f, lo, idx, x);
       return;
     }
   }
@@ -185,15 +217,19 @@ __device__ __forceinline__ typename P::A load_rt(const AT &a, int f, u32 idx)
   return load_raw<P>(a.f[f], a.lo[f], (P::cmask >> f) & 1u, idx);
 }
 // compile-time field index: the residual test folds away
+// (FP16M's 24-bit fields read the whole low-byte record, from the base that field W24_FIRST's
+// pointer holds, so the fields of one stencil point share one record load)
 template <class P, int F, class AT>
 __device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
 {
-  return load_rt<P>(a, F, idx);
+  if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_at<F>(a.f[F], a.lo[W24_FIRST], idx);
+  else return load_rt<P>(a, F, idx);
 }
 template <class P, int F, bool NT, class AT>
 __device__ __forceinline__ typename P::A LO(const AT &a, u32 idx)
 {
-  return load_own<P, NT>(a.f[F], a.lo[F], (P::cmask >> F) & 1u, idx);
+  if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_at<F>(a.f[F], a.lo[W24_FIRST], idx);
+  else return load_own<P, NT>(a.f[F], a.lo[F], (P::cmask >> F) & 1u, idx);
 }
 template <class P, int F, bool NT, class AT>
 __device__ __forceinline__ void ST(const AT &a, u32 idx, typename P::A x)
@@ -254,30 +290,27 @@ __device__ __forceinline__ void coef_block(const AT &a, const void *base, u32 c,
   }
 }
 
-// FP16M medium record of cell c: 4 binary16 values in 8 bytes
+// FP16M medium record of cell c: 4 binary16 values in 8 bytes (mv), 8 in 16 bytes (mt)
+__device__ __forceinline__ float h2f(uint32_t d, int half)
+{
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(d >> (16 * half)));
+}
 __device__ __forceinline__ void med_rec(const void *base, u32 c, float (&v)[4])
 {
   typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
   const dw2_t d = *reinterpret_cast<const dw2_t *>(reinterpret_cast<const char *>(base) + c * 8u);
-  v[0] = (float)__builtin_bit_cast(_Float16, (uint16_t)d.x);
-  v[1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(d.x >> 16));
-  v[2] = (float)__builtin_bit_cast(_Float16, (uint16_t)d.y);
-  v[3] = (float)__builtin_bit_cast(_Float16, (uint16_t)(d.y >> 16));
+  v[0] = h2f(d.x, 0), v[1] = h2f(d.x, 1), v[2] = h2f(d.y, 0), v[3] = h2f(d.y, 1);
 }
-__device__ __forceinline__ float med_r(const void *base, u32 c) { return (float)at<_Float16>(base, c); }
 
-// The velocity half step's 14 coefficients (packed order K_U1X..K_W3Y) of cell c from the
-// FP16M moduli records: per axis the staggered averages of rho, rho_f, C1, C2 over c and its
-// forward neighbour (main.cpp:339-353), then the coefficient formulas of main.cpp:447-481
-// with their zero guards (every W when a C1 average is zero, every U when a rho average is).
+// The velocity half step's 14 coefficients (packed order K_U1X..K_W3Y) of a cell from the
+// FP16M moduli records of the cell (m0) and of its forward neighbours along x, y, z (mn): per
+// axis the staggered averages of rho, rho_f, C1, C2 (main.cpp:339-353), then the coefficient
+// formulas of main.cpp:447-481 with their zero guards (every W when a C1 average is zero,
+// every U when a rho average is).
 template <class AT>
-__device__ __forceinline__ void medium_velocity(const AT &a, u32 c, u32 sy, u32 sz, float (&kv)[14])
+__device__ __forceinline__ void medium_velocity(const AT &a, const float (&m0)[4], const float (&mn)[3][4],
+                                                float (&kv)[14])
 {
-  float m0[4], mn[3][4];
-  med_rec(a.mv, c, m0);
-  med_rec(a.mv, c + 1u, mn[0]);
-  med_rec(a.mv, c + sy, mn[1]);
-  med_rec(a.mv, c + sz, mn[2]);
   bool zw = false, zu = false;
   float W3[3];
 #pragma unroll
@@ -306,26 +339,22 @@ __device__ __forceinline__ void medium_velocity(const AT &a, u32 c, u32 sy, u32 
   for (int k = K_U1X; k <= K_U2Z; ++k) kv[k] = zu ? 0.0f : kv[k];
 }
 
-// The stress half step's 8 coefficients (K_P1..K_SYZ) of cell c: P1, P2, S1, S2, S3 from the
-// cell's C, M, lambda_c, H (main.cpp:483-490), the shear ones from the harmonic mean of mu
-// over each 4-cell quad (main.cpp:326-337), formed from the stored 1/mu: a zero mu is +inf
-// there, so the quad's coefficient is 0 as the reference sets it.
+// The stress half step's 8 coefficients (K_P1..K_SYZ) of cell c from its FP16M record: P1, P2,
+// S1, S2, S3 from C, M, lambda_c, H (main.cpp:483-490), the shear ones from the stored
+// 4-cell harmonic means of mu (main.cpp:326-337).
 template <class AT>
-__device__ __forceinline__ void medium_stress(const AT &a, u32 c, u32 sy, u32 sz, float (&kt)[8])
+__device__ __forceinline__ void medium_stress(const AT &a, u32 c, float (&kt)[8])
 {
-  float t[4];
-  med_rec(a.mt, c, t);
-  const float r0 = med_r(a.mr, c), rx = med_r(a.mr, c + 1u), ry = med_r(a.mr, c + sy);
-  const float rxy = med_r(a.mr, c + 1u + sy), rz = med_r(a.mr, c + sz);
-  const float rxz = med_r(a.mr, c + 1u + sz), ryz = med_r(a.mr, c + sy + sz);
-  kt[0] = a.mc[MC_KP1] * t[0];
-  kt[1] = a.mc[MC_KP2] * t[1];
-  kt[2] = a.mc[MC_KS1] * t[2];
-  kt[3] = a.mc[MC_KS2] * t[3];
-  kt[4] = a.mc[MC_KS3] * t[0];
-  kt[5] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((r0 + rx) + (ry + rxy));
-  kt[6] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((r0 + rx) + (rz + rxz));
-  kt[7] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((r0 + ry) + (rz + ryz));
+  typedef uint32_t dw4_t __attribute__((ext_vector_type(4)));
+  const dw4_t d = *reinterpret_cast<const dw4_t *>(reinterpret_cast<const char *>(a.mt) + c * 16u);
+  kt[0] = a.mc[MC_KP1] * h2f(d.x, 0);
+  kt[1] = a.mc[MC_KP2] * h2f(d.x, 1);
+  kt[2] = a.mc[MC_KS1] * h2f(d.y, 0);
+  kt[3] = a.mc[MC_KS2] * h2f(d.y, 1);
+  kt[4] = a.mc[MC_KS3] * h2f(d.x, 0);
+  kt[5] = a.mc[MC_KSH] * h2f(d.z, 0);
+  kt[6] = a.mc[MC_KSH] * h2f(d.z, 1);
+  kt[7] = a.mc[MC_KSH] * h2f(d.w, 0);
 }
 
 // One folded C-PML update on a carry held in a register: psi = carry + a D;
@@ -442,6 +471,11 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     wp[m] = L<P, FP>(a, col + (u32)(kb - h + 1 + m) * sz);
   }
 
+  // FP16M: the cell's moduli record, rotated along z (the forward-z neighbour's record of one
+  // plane is the cell's own record of the next)
+  float mown[4] = {0, 0, 0, 0};
+  if constexpr (P::medium) med_rec(a.mv, col + (u32)kb * sz, mown);
+
   // Every load of a plane is written before the plane's first store.  The compiler cannot
   // prove the arrays distinct, so it never moves a load above a store; in this order all of
   // a plane's loads (fields, coefficients, C-PML carries) are in flight together and a
@@ -449,32 +483,41 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
   using SC = typename P::SC;
   for (int k = kb; k < ke; ++k) {
     const u32 c = col + (u32)k * sz;
-    wxz[W - 1] = L<P, FTXZ>(a, c + (u32)(h - 1) * sz);
-    wyz[W - 1] = L<P, FTYZ>(a, c + (u32)(h - 1) * sz);
-    wzz[W - 1] = L<P, FTZZ>(a, c + (u32)h * sz);
-    wp[W - 1] = L<P, FP>(a, c + (u32)h * sz);
-    const A txz_c = wxz[h], tyz_c = wyz[h], p_c = wp[h - 1];
-    const A txx_c = L<P, FTXX>(a, c), tyy_c = L<P, FTYY>(a, c), txy_c = L<P, FTXY>(a, c);
-
     // twelve raw staggered differences (kernel.cu:172-185 without the 1/dx factor)
-    A x1 = dfw<P, FTXX, ORDER>(a, c, 1u, txx_c);
-    A x2 = dbw<P, FTXY, ORDER>(a, c, sy, txy_c);
-    A x3 = dwin<ORDER>(wxz);
-    A s1 = dfw<P, FP, ORDER>(a, c, 1u, p_c);
-    A y1 = dfw<P, FTYY, ORDER>(a, c, sy, tyy_c);
-    A y2 = dbw<P, FTXY, ORDER>(a, c, 1u, txy_c);
-    A y3 = dwin<ORDER>(wyz);
-    A s2 = dfw<P, FP, ORDER>(a, c, sy, p_c);
-    A z1 = dwin<ORDER>(wzz);
-    A z2 = dbw<P, FTXZ, ORDER>(a, c, 1u, txz_c);
-    A z3 = dbw<P, FTYZ, ORDER>(a, c, sy, tyz_c);
-    A s3 = dwin<ORDER>(wp);
+    A x1, x2, x3, s1, y1, y2, y3, s2, z1, z2, z3, s3;
+    {
+      wxz[W - 1] = L<P, FTXZ>(a, c + (u32)(h - 1) * sz);
+      wyz[W - 1] = L<P, FTYZ>(a, c + (u32)(h - 1) * sz);
+      wzz[W - 1] = L<P, FTZZ>(a, c + (u32)h * sz);
+      wp[W - 1] = L<P, FP>(a, c + (u32)h * sz);
+      const A txz_c = wxz[h], tyz_c = wyz[h], p_c = wp[h - 1];
+      const A txx_c = L<P, FTXX>(a, c), tyy_c = L<P, FTYY>(a, c), txy_c = L<P, FTXY>(a, c);
+      x1 = dfw<P, FTXX, ORDER>(a, c, 1u, txx_c);
+      x2 = dbw<P, FTXY, ORDER>(a, c, sy, txy_c);
+      x3 = dwin<ORDER>(wxz);
+      s1 = dfw<P, FP, ORDER>(a, c, 1u, p_c);
+      y1 = dfw<P, FTYY, ORDER>(a, c, sy, tyy_c);
+      y2 = dbw<P, FTXY, ORDER>(a, c, 1u, txy_c);
+      y3 = dwin<ORDER>(wyz);
+      s2 = dfw<P, FP, ORDER>(a, c, sy, p_c);
+      z1 = dwin<ORDER>(wzz);
+      z2 = dbw<P, FTXZ, ORDER>(a, c, 1u, txz_c);
+      z3 = dbw<P, FTYZ, ORDER>(a, c, sy, tyz_c);
+      s3 = dwin<ORDER>(wp);
+    }
 
     const A qx0 = LO<P, FQX, (P::nt_v && NT)>(a, c), qy0 = LO<P, FQY, (P::nt_v && NT)>(a, c), qz0 = LO<P, FQZ, (P::nt_v && NT)>(a, c);
     const A vx0 = LO<P, FVX, (P::nt_v && NT)>(a, c), vy0 = LO<P, FVY, (P::nt_v && NT)>(a, c), vz0 = LO<P, FVZ, (P::nt_v && NT)>(a, c);
     A kv[14];  // K[0..13]: U1 xyz, U2 xyz, W1 xyz, W2 xyz, W3x, W3y
-    if constexpr (P::medium) medium_velocity(a, c, sy, sz, kv);
-    else coef_block<P, 14>(a, a.kv, c, 0, kv);
+    float mn[3][4];
+    if constexpr (P::medium) {
+      med_rec(a.mv, c + 1u, mn[0]);
+      med_rec(a.mv, c + sy, mn[1]);
+      med_rec(a.mv, c + sz, mn[2]);
+      medium_velocity(a, mown, mn, kv);
+    } else {
+      coef_block<P, 14>(a, a.kv, c, 0, kv);
+    }
     const A w1x = kv[K_W1X], w2x = kv[K_W2X], w3x = kv[K_W3X];
     const A w1y = kv[K_W1Y], w2y = kv[K_W2Y], w3y = kv[K_W3Y];
     const A w1z = kv[K_W1Z], w2z = kv[K_W2Z];
@@ -548,6 +591,10 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
       wyz[m] = wyz[m + 1];
       wzz[m] = wzz[m + 1];
       wp[m] = wp[m + 1];
+    }
+    if constexpr (P::medium) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mown[q] = mn[2][q];
     }
   }
 }
@@ -627,7 +674,7 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     const A tzz0 = LO<P, FTZZ, (P::nt_t && NT)>(a, c), txy0 = LO<P, FTXY, (P::nt_t && NT)>(a, c), tyz0 = LO<P, FTYZ, (P::nt_t && NT)>(a, c);
     const A txz0 = LO<P, FTXZ, (P::nt_t && NT)>(a, c);
     A kt[8];  // K[14..21]: P1, P2, S1, S2, S3, Sxy, Sxz, Syz
-    if constexpr (P::medium) medium_stress(a, c, sy, sz, kt);
+    if constexpr (P::medium) medium_stress(a, c, kt);
     else coef_block<P, 8>(a, a.kt, c, K_P1, kt);
     const A s1 = kt[K_S1 - K_P1], s2 = kt[K_S2 - K_P1], s3 = kt[K_S3 - K_P1];
     const A kp1 = kt[0], kp2 = kt[K_P2 - K_P1];
@@ -676,13 +723,34 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
 
     // pore pressure and stresses (kernel.cu:484-490)
     const A divw = wx + wy + wz;
-    ST<P, FP, (P::nt_t && NT)>(a, c, p0 - kp1 * (uxx + uyy + uzz) - kp2 * divw);
-    ST<P, FTXX, (P::nt_t && NT)>(a, c, txx0 + s1 * (uyy + uzz) + s2 * uxx + s3 * divw);
-    ST<P, FTYY, (P::nt_t && NT)>(a, c, tyy0 + s1 * (uxx + uzz) + s2 * uyy + s3 * divw);
-    ST<P, FTZZ, (P::nt_t && NT)>(a, c, tzz0 + s1 * (uxx + uyy) + s2 * uzz + s3 * divw);
-    ST<P, FTXY, (P::nt_t && NT)>(a, c, txy0 + kxy * (uxy + uyx));
-    ST<P, FTYZ, (P::nt_t && NT)>(a, c, tyz0 + kyz * (uyz + uzy));
-    ST<P, FTXZ, (P::nt_t && NT)>(a, c, txz0 + kxz * (uxz + uzx));
+    const A pn = p0 - kp1 * (uxx + uyy + uzz) - kp2 * divw;
+    const A txxn = txx0 + s1 * (uyy + uzz) + s2 * uxx + s3 * divw;
+    const A tyyn = tyy0 + s1 * (uxx + uzz) + s2 * uyy + s3 * divw;
+    const A tzzn = tzz0 + s1 * (uxx + uyy) + s2 * uzz + s3 * divw;
+    const A txyn = txy0 + kxy * (uxy + uyx);
+    const A tyzn = tyz0 + kyz * (uyz + uzy);
+    const A txzn = txz0 + kxz * (uxz + uzx);
+    if constexpr (P::w24) {
+      // upper halves to the field arrays, the seven low bytes in one record store
+      const u32 r[7] = {w24_round(txxn), w24_round(tyyn), w24_round(tzzn), w24_round(txyn),
+                        w24_round(txzn), w24_round(tyzn), w24_round(pn)};
+      at<uint16_t>(a.f[FTXX], c) = (uint16_t)(r[0] >> 16);
+      at<uint16_t>(a.f[FTYY], c) = (uint16_t)(r[1] >> 16);
+      at<uint16_t>(a.f[FTZZ], c) = (uint16_t)(r[2] >> 16);
+      at<uint16_t>(a.f[FTXY], c) = (uint16_t)(r[3] >> 16);
+      at<uint16_t>(a.f[FTXZ], c) = (uint16_t)(r[4] >> 16);
+      at<uint16_t>(a.f[FTYZ], c) = (uint16_t)(r[5] >> 16);
+      at<uint16_t>(a.f[FP], c) = (uint16_t)(r[6] >> 16);
+      at<dw2_t>(a.lo[W24_FIRST], c) = w24_record(r);
+    } else {
+      ST<P, FP, (P::nt_t && NT)>(a, c, pn);
+      ST<P, FTXX, (P::nt_t && NT)>(a, c, txxn);
+      ST<P, FTYY, (P::nt_t && NT)>(a, c, tyyn);
+      ST<P, FTZZ, (P::nt_t && NT)>(a, c, tzzn);
+      ST<P, FTXY, (P::nt_t && NT)>(a, c, txyn);
+      ST<P, FTYZ, (P::nt_t && NT)>(a, c, tyzn);
+      ST<P, FTXZ, (P::nt_t && NT)>(a, c, txzn);
+    }
 
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) {
@@ -813,15 +881,12 @@ __global__ void k_set_counter(int *itp, int v)
   if (threadIdx.x == 0) atomicExch(itp, v);
 }
 
-// In-place multiply of an FP16M 24-bit array (upper 16 bits in p, the next byte in lo).
-__global__ void k_rescale24(uint16_t *p, uint8_t *lo, int64_t n, double f)
+// In-place multiply of an FP16M 24-bit field (lo: its byte of record 0).
+__global__ void k_rescale24(void *p, int8_t *lo, int64_t n, double f)
 {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const u32 r = w24_round(w24_load(p[i], lo[i]) * (float)f);
-    p[i] = (uint16_t)(r >> 16);
-    lo[i] = (uint8_t)(r >> 8);
-  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    w24_store(p, lo, (u32)i, w24_load(p, lo, (u32)i) * (float)f);
 }
 
 // In-place multiply of a stored array (re-expresses values when a storage scale changes).
@@ -1045,17 +1110,22 @@ hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hi
     case PREC_FP16T:
       hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, lo, n, f);
       break;
-    case PREC_FP16M:
-      if (lo)
-        hipLaunchKernelGGL(k_rescale24, dim3(blocks), dim3(256), 0, s, (uint16_t *)p, (uint8_t *)lo, n, f);
-      else
-        hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, lo, n, f);
+    case PREC_FP16M:  // the binary16 arrays (v, q); launch_rescale24 takes tau and p
+      hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, lo, n, f);
       break;
     case PREC_FP64:
       hipLaunchKernelGGL(k_rescale<double>, dim3(blocks), dim3(256), 0, s, (double *)p, lo, n, f);
       break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_rescale24(void *p, int8_t *lo, int64_t n, double f, hipStream_t s)
+{
+  if (n <= 0 || !p || !lo) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_rescale24, dim3(blocks), dim3(256), 0, s, p, lo, n, f);
   return hipGetLastError();
 }
 

@@ -163,7 +163,7 @@ extern "C" int biot_run_stream(biot_handle *h, int32_t nsteps, int32_t chunk, fl
     if (rc) break;
     if (next_snap < nsnap && b == it0 + snap_steps[next_snap] + 1) {
       float *ds = d_snap + (size_t)next_snap * slice;
-      if (launch_slice_xz(h->prec, h->field_ptr(field), h->lo_ptr(field), h->lo_ptr(field) != nullptr,
+      if (launch_slice_xz(h->prec, h->field_ptr(field), h->lo_ptr(field), ((comp_fields(h->prec) >> field) & 1u) != 0,
                           (double)h->fscale[field], j, h->g.nx, h->g.ny, h->g.nz, ds, h->stream) != hipSuccess) {
         rc = BIOT_ERR_HIP;
         break;
