@@ -218,6 +218,14 @@ int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every);
  * handles), reduced in FP64 on the device; residual (may be NULL) receives d - observed in
  * the trace layout. */
 int biot_misfit_l2(biot_handle *h, const float *observed, float *residual, double *J);
+/* No counterpart in the reference: biot_misfit_l2 without device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The observed and residual staging and the FP64 accumulator live in one static
+ * 64 MiB device array per GPU, held until the process ends; a trace layout of more than
+ * BIOT_MISFIT_NOALLOC_MAX_SAMPLES samples (ncomp * nt_max * nrec) returns BIOT_ERR_ARG.  Every
+ * call on a GPU uses that GPU's one array: calls that overlap on one GPU race. */
+#define BIOT_MISFIT_NOALLOC_MAX_SAMPLES 8388576
+int biot_misfit_l2_noalloc(biot_handle *h, const float *observed, float *residual, double *J);
 /* Discrete adjoint of the checkpointed run.  adj_source = dJ/d(trace samples), trace
  * layout.  Segments of `every` steps are recomputed from their checkpoints and swept
  * backward; coefficient gradients are summed in FP64 on the device.  grad_params (may be

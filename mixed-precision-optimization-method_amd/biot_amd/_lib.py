@@ -143,6 +143,7 @@ def lib():
         "biot_run_stream": ([H, i32, i32, FP, i32, i32, C.POINTER(C.c_int32), i32, FP], C.c_int),
         "biot_forward_checkpointed": ([H, i32, i32], C.c_int),
         "biot_misfit_l2": ([H, FP, FP, _DP], C.c_int),
+        "biot_misfit_l2_noalloc": ([H, FP, FP, _DP], C.c_int),
         "biot_adjoint": ([H, FP, C.POINTER(_DP), C.POINTER(_DP)], C.c_int),
         "biot_params_coefficients": ([C.POINTER(Grid), C.POINTER(Params), C.POINTER(_DP)], C.c_int),
         "biot_params_chain": ([C.POINTER(Grid), C.POINTER(Params), C.POINTER(_DP), C.POINTER(_DP)],
@@ -188,7 +189,7 @@ def exported_symbols():
             "biot_write_text_xz", "biot_write_binary", "biot_run_stream_traces", "biot_run_stream",
             "biot_set_params",
             "biot_set_medium_device", "biot_kernel_coefficients", "biot_read_kernel_coefficients",
-            "biot_forward_checkpointed", "biot_misfit_l2", "biot_adjoint",
+            "biot_forward_checkpointed", "biot_misfit_l2", "biot_misfit_l2_noalloc", "biot_adjoint",
             "biot_params_coefficients", "biot_params_chain", "biot_batch_create",
             "biot_batch_destroy", "biot_batch_last_error", "biot_batch_add_shot",
             "biot_batch_shot_count", "biot_batch_set_model", "biot_batch_shot_handle",
