@@ -212,7 +212,10 @@ typedef struct biot_params {
 } biot_params;
 int biot_set_params(biot_handle *h, const biot_params *p);
 /* Resets, runs nt steps recording traces (receivers must hold >= nt steps) and keeps the
- * state every `every` steps on the device for the adjoint.  FP32, FP64 and FP16 handles. */
+ * state every `every` steps on the device for the adjoint.  FP32, FP64 and FP16 handles.
+ * every = 1 keeps every step's state (nt + 1 states in one device block, BIOT_ERR_NOMEM if
+ * it does not fit): biot_adjoint then reads them instead of recomputing each segment, and
+ * gives the same gradients as every = 50 bit for bit. */
 int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every);
 /* J = 1/2 sum (d - observed)^2 over the recorded traces (the double traces on FP64
  * handles), reduced in FP64 on the device; residual (may be NULL) receives d - observed in

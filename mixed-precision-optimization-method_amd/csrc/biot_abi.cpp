@@ -580,8 +580,9 @@ extern "C" int biot_reset(biot_handle *h)
   return BIOT_OK;
 }
 
-int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks)
+int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks, char *states)
 {
+  const size_t sb = states ? state_bytes() : 0;
   SourceArgs sa;
   source_args(sa);
   RecordArgs ra;
@@ -596,6 +597,7 @@ int biot_handle::enqueue_steps(int nsteps, hipEvent_t *marks)
       hipError_t e = launch_source(prec, sa, stream);
       if (e != hipSuccess) return fail(this, BIOT_ERR_HIP, "source launch: %s", hipGetErrorString(e));
     }
+    if (states) save_state(states + (size_t)s * sb);
     if (marks) (void)hipEventRecord(marks[3 * s], stream);
     hipError_t e = launch_velocity(prec, g.order, args, shape, stream);
     if (e != hipSuccess) return fail(this, BIOT_ERR_HIP, "velocity launch: %s", hipGetErrorString(e));

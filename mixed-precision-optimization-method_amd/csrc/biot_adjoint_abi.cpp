@@ -15,6 +15,11 @@
 
 using namespace biot;
 
+// with every step's state stored, the float partial gradient sums are added into the FP64
+// sums every this many adjoint steps (the default checkpoint spacing: at most 50 float
+// additions per partial sum)
+constexpr int kStoredFlushEvery = 50;
+
 namespace {
 
 int afail(biot_handle *h, int code, const char *fmt, ...)
@@ -97,8 +102,30 @@ extern "C" int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t eve
     if ((h->src_mask >> f) & 1u && !(f >= 6))
       return afail(h, BIOT_ERR_ARG, "the adjoint path supports sources on stresses and pressure only");
   h->free_adjoint();
-  const int nseg = (nt + every - 1) / every;
   const size_t sb = h->state_bytes();
+  if (every == 1) {
+    // every step's state, in one block the adjoint reads directly (no recomputation)
+    void *p = nullptr;
+    const size_t bytes = sb * (size_t)(nt + 1);
+    if (hipMalloc(&p, bytes) != hipSuccess)
+      return afail(h, BIOT_ERR_NOMEM, "the %d step states (%zu bytes) do not fit; use every > 1", nt + 1, bytes);
+    h->ckpt.push_back(p);
+    int rc = biot_reset(h);
+    if (rc) return rc;
+    AHIP(h, hipEventRecord(h->ev0, h->stream));
+    rc = h->enqueue_steps(nt, nullptr, (char *)p);
+    if (rc) return rc;
+    h->save_state((char *)p + (size_t)nt * sb);
+    AHIP(h, hipEventRecord(h->ev1, h->stream));
+    AHIP(h, hipEventSynchronize(h->ev1));
+    float ms = 0;
+    AHIP(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    h->ckpt_every = 1;
+    h->ckpt_nt = nt;
+    return BIOT_OK;
+  }
+  const int nseg = (nt + every - 1) / every;
   for (int s = 0; s < nseg; ++s) {
     void *p = nullptr;
     if (hipMalloc(&p, sb) != hipSuccess) {
@@ -163,7 +190,8 @@ extern "C" int biot_adjoint(biot_handle *h, const float *adj_source, double *con
   const int nt = h->ckpt_nt, every = h->ckpt_every, order = h->g.order;
   const int64_t n = h->n;
   const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
-  int rc = h->alloc_adjoint(every);
+  const bool stored = every == 1;  // every step's state kept by biot_forward_checkpointed
+  int rc = h->alloc_adjoint(stored ? 0 : every);
   if (rc) return rc;
   AHIP(h, hipMemsetAsync(h->d_lam, 0, h->lam_bytes(), h->stream));
   AHIP(h, hipMemsetAsync(h->d_grad, 0, (size_t)NKC * n * sizeof(double), h->stream));
@@ -183,7 +211,18 @@ extern "C" int biot_adjoint(biot_handle *h, const float *adj_source, double *con
 
   // ---- backward sweep over the checkpoint segments
   AHIP(h, hipEventRecord(h->ev0, h->stream));
-  const int nseg = (int)h->ckpt.size();
+  if (stored) {
+    // the stored states are the segment states: state it is slot it.  The float partial sums
+    // go into the FP64 sums at the same steps as with 50-step segments (after it = 950, ...,
+    // 0), so both modes give the same gradients bit for bit.
+    A.slots = (const char *)h->ckpt[0];
+    for (int it = nt - 1; it >= 0; --it) {
+      AHIP(h, launch_adjoint_inject(h->prec, I, it, h->stream));
+      AHIP(h, launch_adjoint_step(h->prec, order, A, it, h->stream));
+      if (it % kStoredFlushEvery == 0) AHIP(h, launch_adjoint_flush(h->prec, order, A, h->stream));
+    }
+  }
+  const int nseg = stored ? 0 : (int)h->ckpt.size();
   for (int s = nseg - 1; s >= 0; --s) {
     const int n0 = s * every, K = std::min(every, nt - n0);
     h->load_state(h->ckpt[s]);

@@ -129,7 +129,7 @@ int biot_handle::alloc_adjoint(int every)
     }
     return true;
   };
-  if (!get(&d_slots, state_bytes() * (size_t)(every + 1)) || !get(&d_lam, lam_bytes()) ||
+  if ((every > 0 && !get(&d_slots, state_bytes() * (size_t)(every + 1))) || !get(&d_lam, lam_bytes()) ||
       !get((void **)&d_grad, (size_t)NKC * n * sizeof(double)) ||
       (prec != BIOT_FP64 && !get((void **)&d_gpart, (size_t)NKC * n * sizeof(float))) ||
       !get((void **)&d_adjsrc, (size_t)std::max<int64_t>(m, 1) * sizeof(float))) {

@@ -96,6 +96,9 @@ struct biot_handle {
   std::vector<double> K_last[biot::NKC];     // kernel coefficients built from par
   std::vector<void *> ckpt;                  // checkpoint states, every ckpt_every steps
   int ckpt_every = 0, ckpt_nt = 0;
+  // ckpt_every == 1: ckpt[0] is one block of ckpt_nt + 1 states, state it being step it's
+  // state after its source (the last one the state after the run), which the adjoint reads
+  // in place of segment recomputation
   void *d_slots = nullptr;                   // segment states (ckpt_every + 1)
   void *d_lam = nullptr;                     // adjoint fields, carries, derivative adjoints
   double *d_grad = nullptr;                  // 22 FP64 coefficient gradients
@@ -107,8 +110,9 @@ struct biot_handle {
   void load_state(const void *src);
   void free_adjoint();
   void forget_model();  // drop parameter-built coefficients and every adjoint buffer
-  // marks: null, or 3 events per step recorded before velocity, between, after stress
-  int enqueue_steps(int nsteps, hipEvent_t *marks);
+  // marks: null, or 3 events per step recorded before velocity, between, after stress;
+  // states: null, or room for nsteps states, step s's state saved after its source
+  int enqueue_steps(int nsteps, hipEvent_t *marks, char *states = nullptr);
 
   // argument blocks shared by the single-grid and batched drivers
   void source_args(biot::SourceArgs &sa) const;
@@ -116,7 +120,7 @@ struct biot_handle {
   int clear_nonfinite();                     // enqueue a clear of d_flag on the stream
   int check_nonfinite();                     // synchronize; BIOT_ERR_NONFINITE if it was set
   size_t lam_bytes() const;                  // adjoint wavefields, carries, derivative adjoints
-  int alloc_adjoint(int every);              // d_slots (every+1 states), d_lam, d_grad, d_adjsrc
+  int alloc_adjoint(int every);              // d_slots (every+1 states; none for 0), d_lam, d_grad, d_adjsrc
   void adj_args(biot::AdjArgs &A) const;     // after alloc_adjoint
   void inject_args(biot::InjectArgs &I) const;
 };

@@ -95,3 +95,7 @@ def test_adjoint_bytes_closed_form():
     total, per_step, u = bench.adjoint_bytes(nx, ny, nz, order, pml, nt, every)
     assert u == upd and total == want
     assert per_step == 2 * state + v + t + 700 * upd + 96 * memb
+    # every step's state stored: the adjoint passes only, a flush every 50 steps (37 steps: one)
+    total, per_step, u = bench.adjoint_bytes(nx, ny, nz, order, pml, nt, 1)
+    assert per_step == (49 + 24 + 76 + 26) * 4 * upd + 24 * 4 * memb
+    assert total == nt * per_step + 22 * (4 + 4 + 8 + 8) * upd

@@ -10,6 +10,8 @@
   transposed differences, its own dot-product tests on the CPU) on a 24^3 perturbed case:
   traces 1e-9, parameter gradients 1e-8 (rel L2; the two differ only in summation order).
 * The torch.autograd binding returns the same gradients as the direct call.
+* Every step's state stored (every = 1, no segment recomputation) gives the gradients of
+  50-step checkpoints bit for bit.
 """
 import math
 
@@ -169,6 +171,33 @@ def test_fp64_adjoint_matches_host_adjoint():
     print(f"  traces rel L2 {et:.2e}; gradients " + " ".join(f"{k} {e:.2e}" for k, e in eg.items()))
     assert et <= 1e-9
     assert max(eg.values()) <= 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp64"])
+def test_stored_states_equal_recomputed_segments(precision):
+    cfg = configs.adjoint128(n=48, nt=260, every=50, sphere=False)
+    nt = cfg["nt"]
+    print(f"stored states vs 50-step segments {precision} 48^3 {nt} steps", flush=True)
+    p0 = _start_params(cfg)
+    s = configs.make_solver(cfg, precision=precision)
+    s.set_params(p0)
+    y = None
+    out = {}
+    for every in (50, 1):
+        s.forward_checkpointed(nt, every)
+        tr = s.traces()
+        if y is None:
+            y = np.random.default_rng(5).standard_normal(tr.shape).astype(np.float32)
+        g, gk = s.adjoint(y, params=True, coefs=True)
+        out[every] = (tr, g, gk)
+        print(f"  every {every}: {s.last_run_ms():.1f} ms adjoint", flush=True)
+    assert np.array_equal(out[50][0], out[1][0])
+    for k in PARAM_NAMES:
+        assert np.array_equal(out[50][1][k], out[1][1][k]), k
+    for a, b in zip(out[50][2], out[1][2]):
+        assert np.array_equal(a, b)
+    assert any(np.abs(out[1][1][k]).max() > 0 for k in PARAM_NAMES)
+    s.close()
 
 
 def test_autograd_binding_matches_direct_adjoint():
