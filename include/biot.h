@@ -212,7 +212,8 @@ typedef struct biot_params {
 } biot_params;
 int biot_set_params(biot_handle *h, const biot_params *p);
 /* Resets, runs nt steps recording traces (receivers must hold >= nt steps) and keeps the
- * state every `every` steps on the device for the adjoint.  FP32, FP64 and FP16 handles.
+ * state every `every` steps on the device for the adjoint.  FP32, FP64, FP16 and FP16T handles
+ * (FP16T states carry their residuals, and the adjoint reads them).
  * every = 1 keeps every step's state (nt + 1 states in one device block, BIOT_ERR_NOMEM if
  * it does not fit): biot_adjoint then reads them instead of recomputing each segment, and
  * gives the same gradients as every = 50 bit for bit. */
@@ -253,10 +254,12 @@ int biot_params_chain(const biot_grid *g, const biot_params *p, const double *co
 
 /* ---- batched shots (no counterpart in the reference) ---- */
 /* Many shots, each a window of one model grid with its own absorbing boundary, source and
- * receivers, advanced together: one persistent launch per half step covers the tiles of
- * every shot, and runs of steps are captured into a HIP graph and replayed.  The gradient
- * sums every shot's parameter gradient into one model-grid volume on the device.
- * Precisions FP32, FP16 (binary16 wavefields, float arithmetic, FP64 gradient sums), FP64. */
+ * receivers, advanced together: one launch per half step (a grid of one block per tile
+ * unless a persistent grid is set) covers the tiles of every shot; runs of steps are
+ * captured into a HIP graph and replayed.  The gradient sums every shot's parameter
+ * gradient into one model-grid volume on the device.  Precisions FP32, FP16 (binary16
+ * wavefields, float arithmetic, FP64 gradient sums), FP16T (FP16 with the 8-bit residual on
+ * tau and p), FP64. */
 typedef struct biot_batch biot_batch;
 typedef struct biot_batch_config {
   int32_t nx, ny, nz;        /* model grid the windows are cut from (gradient volume) */

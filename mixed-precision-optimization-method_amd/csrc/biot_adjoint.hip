@@ -59,8 +59,28 @@ template <typename S>
 struct AdjT {
   using A = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
   using C = typename std::conditional<std::is_same<S, float>::value, float, S>::type;  // forward carry
+  using T = S;  // stored element of fields and coefficients
   static constexpr bool scaled = std::is_same<S, _Float16>::value;
+  static constexpr bool res = false;
 };
+// FP16T states: binary16 fields and coefficients, an int8 residual on tau and p (value =
+// h + r ulp(h)/256, as the forward kernels store it), float C-PML carries
+struct H16R {};
+template <>
+struct AdjT<H16R> {
+  using A = float;
+  using C = float;
+  using T = _Float16;
+  static constexpr bool scaled = true;
+  static constexpr bool res = true;
+};
+// the residual unit ulp(h)/256 = 2^(e-33), e the biased exponent of h (subnormals e = 1)
+__device__ __forceinline__ float res_unit(_Float16 h)
+{
+  const uint32_t u = __builtin_bit_cast(uint16_t, h);
+  const int e = max((int)((u >> 10) & 31u), 1);
+  return __builtin_bit_cast(float, (uint32_t)(e + 94) << 23);
+}
 
 struct CellIdx {
   int i, j, k;
@@ -90,12 +110,27 @@ __device__ __forceinline__ const void *fw_carry(const AT &a, int q, int ax)
 {
   return a.slots + (uint64_t)q * a.sb + a.coff[ax];
 }
+// the residuals of field f in state q (FP16T tau and p), or null
+template <class AT>
+__device__ __forceinline__ const int8_t *fw_lo(const AT &a, int q, int f)
+{
+  if (!a.loff || !((a.lomask >> f) & 1u)) return nullptr;
+  return reinterpret_cast<const int8_t *>(a.slots + (uint64_t)q * a.sb + a.loff +
+                                          (uint64_t)f * (uint64_t)a.nx * a.ny * a.nz);
+}
 
 template <typename S, int ORDER>
-__device__ __forceinline__ typename AdjT<S>::A diff(const void *f, u32 c, u32 s, int fwd)
+__device__ __forceinline__ typename AdjT<S>::A diff(const void *f, const int8_t *lo, u32 c, u32 s, int fwd)
 {
   using A = typename AdjT<S>::A;
-  auto L = [&](u32 idx) { return (A)at<S>(f, idx); };
+  using T = typename AdjT<S>::T;
+  auto L = [&](u32 idx) {
+    const T h = at<T>(f, idx);
+    if constexpr (AdjT<S>::res) {
+      if (lo) return (A)h + (A)lo[idx] * res_unit(h);
+    }
+    return (A)h;
+  };
   if (fwd) {
     if constexpr (ORDER == 2) return L(c + s) - L(c);
     return A(9.0 / 8.0) * (L(c + s) - L(c)) + A(-1.0 / 24.0) * (L(c + 2 * s) - L(c - s));
@@ -157,12 +192,13 @@ __device__ __forceinline__ void kblock(const AT &a, const void *base, u32 c, int
                                        typename AdjT<S>::A (&out)[N])
 {
   using A = typename AdjT<S>::A;
-  uint32_t d[packed::block_dwords<S, N>()];
-  packed::load<S, N, true>(base, (uint64_t)a.nx * a.ny * a.nz, c, d);
+  using T = typename AdjT<S>::T;
+  uint32_t d[packed::block_dwords<T, N>()];
+  packed::load<T, N, true>(base, (uint64_t)a.nx * a.ny * a.nz, c, d);
 #pragma unroll
   for (int e = 0; e < N; ++e) {
-    if constexpr (AdjT<S>::scaled) out[e] = (A)packed::value<S>(d, e) * (A)a.ks[k0 + e];
-    else out[e] = (A)packed::value<S>(d, e);
+    if constexpr (AdjT<S>::scaled) out[e] = (A)packed::value<T>(d, e) * (A)a.ks[k0 + e];
+    else out[e] = (A)packed::value<T>(d, e);
   }
 }
 // the N gradient sums of K[k0..k0+N) at cell c: float partial sums of the packed array p on
@@ -210,7 +246,7 @@ __device__ __forceinline__ void pml_load(const AT &a, const Der &D, const CellId
 {
   using A = typename AdjT<S>::A;
   const u32 st[3] = {1u, (u32)a.sy, (u32)a.sz};
-  r.E = diff<S, ORDER>(fw_field(a, qf, D.field), ci.c, st[D.dir], D.fwd);
+  r.E = diff<S, ORDER>(fw_field(a, qf, D.field), fw_lo(a, qf, D.field), ci.c, st[D.dir], D.fwd);
   r.ix = carry_idx(a, D.dir, D.slot, ci);
   r.e = r.aa = r.cf = r.lc = A(0);
   if (r.ix >= 0) {
@@ -339,8 +375,8 @@ __device__ __forceinline__ void adj_velocity_local(const AT &a, int q, u32 t)
     lq[m] = at<A>(a.lam[3 + m], c);
     U1[m] = kv[kU1[m]], U2[m] = kv[kU2[m]];
     W1[m] = kv[kW1[m]], W2[m] = kv[kW2[m]], W3[m] = kv[kW3[m]];
-    q0[m] = (double)at<S>(fw_field(a, q, 3 + m), c);
-    q1[m] = (double)at<S>(fw_field(a, q + 1, 3 + m), c);
+    q0[m] = (double)at<typename AdjT<S>::T>(fw_field(a, q, 3 + m), c);  // q carries no residual
+    q1[m] = (double)at<typename AdjT<S>::T>(fw_field(a, q + 1, 3 + m), c);
   }
   PmlIn<A> in[12];
 #pragma unroll
@@ -704,6 +740,7 @@ hipError_t passes(int prec, int order, F f, int first, int last)
     if (prec == PREC_FP32) e = order == 2 ? f.template run<float, 2>(which) : f.template run<float, 4>(which);
     else if (prec == PREC_FP64) e = order == 2 ? f.template run<double, 2>(which) : f.template run<double, 4>(which);
     else if (prec == PREC_FP16) e = order == 2 ? f.template run<_Float16, 2>(which) : f.template run<_Float16, 4>(which);
+    else if (prec == PREC_FP16T) e = order == 2 ? f.template run<H16R, 2>(which) : f.template run<H16R, 4>(which);
     else return hipErrorInvalidValue;
     if (e != hipSuccess) return e;
   }
@@ -741,7 +778,7 @@ hipError_t launch_adjoint_inject(int prec, const InjectArgs &a, int it, hipStrea
   if (a.nrec <= 0 || a.ncomp <= 0) return hipSuccess;
   dim3 grid((unsigned)((a.nrec + 255) / 256), (unsigned)a.ncomp);
   if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject<double>, grid, dim3(256), 0, s, a, it);
-  else if (prec == PREC_FP32 || prec == PREC_FP16)
+  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T)
     hipLaunchKernelGGL(k_adj_inject<float>, grid, dim3(256), 0, s, a, it);
   else return hipErrorInvalidValue;
   return hipGetLastError();
@@ -753,7 +790,7 @@ hipError_t launch_adjoint_inject_batch(int prec, const InjectArgs *tab, int nsho
   if (nshots <= 0 || max_nrec <= 0 || max_ncomp <= 0) return hipSuccess;
   dim3 grid((unsigned)((max_nrec + 255) / 256), (unsigned)max_ncomp, (unsigned)nshots);
   if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject_batch<double>, grid, dim3(256), 0, s, tab, it);
-  else if (prec == PREC_FP32 || prec == PREC_FP16)
+  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T)
     hipLaunchKernelGGL(k_adj_inject_batch<float>, grid, dim3(256), 0, s, tab, it);
   else return hipErrorInvalidValue;
   return hipGetLastError();

@@ -170,8 +170,9 @@ extern "C" int biot_batch_create(const biot_batch_config *c, biot_batch **out)
   if (c->nx < 8 || c->ny < 8 || c->nz < 8)
     return refuse(nullptr, BIOT_ERR_ARG, "the model grid must be at least 8 cells on every axis");
   if (c->order != 2 && c->order != 4) return refuse(nullptr, BIOT_ERR_ARG, "order must be 2 or 4");
-  if (c->precision != BIOT_FP32 && c->precision != BIOT_FP16 && c->precision != BIOT_FP64)
-    return refuse(nullptr, BIOT_ERR_ARG, "batches run BIOT_FP32, BIOT_FP16 or BIOT_FP64");
+  if (c->precision != BIOT_FP32 && c->precision != BIOT_FP16 && c->precision != BIOT_FP16T &&
+      c->precision != BIOT_FP64)
+    return refuse(nullptr, BIOT_ERR_ARG, "batches run BIOT_FP32, BIOT_FP16, BIOT_FP16T or BIOT_FP64");
   if (!(c->dx > 0) || !(c->dt > 0) || c->npml < 0)
     return refuse(nullptr, BIOT_ERR_ARG, "dx and dt must be positive and npml >= 0");
   int ndev = 0;
@@ -581,14 +582,18 @@ int gradient_group(biot_batch *b, int s0, int s1, int nt, int every, const float
     BHIP(b, hipMemcpy(b->d_blocks, bs.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->d_inj, inj.data(), S * sizeof(InjectArgs), hipMemcpyHostToDevice));
   }
-  // copy tables: [save ckpt k] nseg, [load ckpt k] nseg, [save slot q] every + 1; 4 pieces per shot
-  const int per = 4 * S;
+  // copy tables: [save ckpt k] nseg, [load ckpt k] nseg, [save slot q] every + 1; 5 pieces per shot
+  // (fields, the three carry blocks, the FP16T residuals: 0 bytes on the other policies)
+  constexpr int NP = 5;
+  const int per = NP * S;
   auto pieces = [&](biot_handle *h, char *state, bool to_state, CopyEntry *e) {
-    char *live[4] = {(char *)h->d_fields, (char *)h->d_carry[0], (char *)h->d_carry[1], (char *)h->d_carry[2]};
-    const size_t bytes[4] = {(size_t)NFIELD * h->n * h->esz, (size_t)NCARRY * h->slab[0] * h->csz,
-                             (size_t)NCARRY * h->slab[1] * h->csz, (size_t)NCARRY * h->slab[2] * h->csz};
-    for (int p = 0; p < 4; ++p) {
-      char *st = state + h->state_off(p);
+    char *live[NP] = {(char *)h->d_fields, (char *)h->d_carry[0], (char *)h->d_carry[1], (char *)h->d_carry[2],
+                      (char *)h->d_lo};
+    const size_t bytes[NP] = {(size_t)NFIELD * h->n * h->esz, (size_t)NCARRY * h->slab[0] * h->csz,
+                              (size_t)NCARRY * h->slab[1] * h->csz, (size_t)NCARRY * h->slab[2] * h->csz,
+                              h->d_lo ? h->lo_bytes() : 0};
+    for (int p = 0; p < NP; ++p) {
+      char *st = state + (p < 4 ? h->state_off(p) : h->state_lo_off());
       e[p].src = to_state ? (const void *)live[p] : (const void *)st;
       e[p].dst = to_state ? (void *)st : (void *)live[p];
       e[p].bytes = bytes[p];
@@ -599,11 +604,11 @@ int gradient_group(biot_batch *b, int s0, int s1, int nt, int every, const float
     for (int i = 0; i < S; ++i) {
       biot_handle *h = b->shots[s0 + i].h;
       for (int k = 0; k < nseg; ++k) {
-        pieces(h, (char *)b->shots[s0 + i].ckpt[k], true, &tab[(size_t)k * per + 4 * i]);
-        pieces(h, (char *)b->shots[s0 + i].ckpt[k], false, &tab[(size_t)(nseg + k) * per + 4 * i]);
+        pieces(h, (char *)b->shots[s0 + i].ckpt[k], true, &tab[(size_t)k * per + NP * i]);
+        pieces(h, (char *)b->shots[s0 + i].ckpt[k], false, &tab[(size_t)(nseg + k) * per + NP * i]);
       }
       for (int q = 0; q <= every; ++q)
-        pieces(h, (char *)h->d_slots + (size_t)q * h->state_bytes(), true, &tab[(size_t)(2 * nseg + q) * per + 4 * i]);
+        pieces(h, (char *)h->d_slots + (size_t)q * h->state_bytes(), true, &tab[(size_t)(2 * nseg + q) * per + NP * i]);
     }
     BHIP(b, hipMalloc((void **)&b->d_copy, tab.size() * sizeof(CopyEntry)));
     BHIP(b, hipMemcpy(b->d_copy, tab.data(), tab.size() * sizeof(CopyEntry), hipMemcpyHostToDevice));

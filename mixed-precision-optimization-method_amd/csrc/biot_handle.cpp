@@ -12,11 +12,13 @@
 using namespace biot;
 
 // One time-stepping state (wavefields + C-PML carries) in one device buffer: the fields
-// block at offset 0, then the three carry blocks, each piece starting on a 256-byte boundary.
+// block at offset 0, the residual / low-byte block of the 16-bit policies that have one, then
+// the three carry blocks, each piece starting on a 256-byte boundary.
 static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+size_t biot_handle::state_lo_off() const { return round256((size_t)NFIELD * n * esz); }
 size_t biot_handle::state_off(int piece) const
 {
-  size_t off = round256((size_t)NFIELD * n * esz);
+  size_t off = state_lo_off() + round256(lo_bytes());
   for (int ax = 0; ax < piece - 1; ++ax) off += round256((size_t)NCARRY * slab[ax] * csz);
   return piece == 0 ? 0 : off;
 }
@@ -26,6 +28,7 @@ void biot_handle::save_state(void *dst)
 {
   char *p = (char *)dst;
   (void)hipMemcpyAsync(p, d_fields, (size_t)NFIELD * n * esz, hipMemcpyDeviceToDevice, stream);
+  if (d_lo) (void)hipMemcpyAsync(p + state_lo_off(), d_lo, lo_bytes(), hipMemcpyDeviceToDevice, stream);
   for (int ax = 0; ax < 3; ++ax) {
     const size_t b = (size_t)NCARRY * slab[ax] * csz;
     if (b) (void)hipMemcpyAsync(p + state_off(1 + ax), d_carry[ax], b, hipMemcpyDeviceToDevice, stream);
@@ -36,6 +39,7 @@ void biot_handle::load_state(const void *src)
 {
   const char *p = (const char *)src;
   (void)hipMemcpyAsync(d_fields, p, (size_t)NFIELD * n * esz, hipMemcpyDeviceToDevice, stream);
+  if (d_lo) (void)hipMemcpyAsync(d_lo, p + state_lo_off(), lo_bytes(), hipMemcpyDeviceToDevice, stream);
   for (int ax = 0; ax < 3; ++ax) {
     const size_t b = (size_t)NCARRY * slab[ax] * csz;
     if (b) (void)hipMemcpyAsync(d_carry[ax], p + state_off(1 + ax), b, hipMemcpyDeviceToDevice, stream);
@@ -147,6 +151,8 @@ void biot_handle::adj_args(AdjArgs &A) const
   A.slots = (const char *)d_slots;
   A.sb = state_bytes();
   A.fbytes = (uint64_t)n * esz;
+  A.loff = prec == BIOT_FP16T ? state_lo_off() : 0;  // FP16T residuals (the adjoint runs no other 16-bit policy)
+  A.lomask = prec == BIOT_FP16T ? comp_fields(prec) : 0u;
   for (int ax = 0; ax < 3; ++ax) A.coff[ax] = state_off(1 + ax);
   char *lp = (char *)d_lam;
   for (int f = 0; f < NFIELD; ++f) A.lam[f] = lp + (size_t)f * n * asz;

@@ -106,6 +106,7 @@ struct biot_handle {
   float *d_adjsrc = nullptr;                 // adjoint source traces
   size_t state_bytes() const;
   size_t state_off(int piece) const;  // 0 fields, 1..3 carries of x, y, z, 4 total
+  size_t state_lo_off() const;        // the residual / low-byte block (lo_bytes())
   void save_state(void *dst);
   void load_state(const void *src);
   void free_adjoint();

@@ -72,28 +72,38 @@ def test_batch16_forward_equals_single_grids(cfg):
 
 
 def test_batch16_fp16_gradient_against_fp32(cfg):
-    """Binary16 wavefields (the reference's `half` storage) carry ~2e-2 relative trace error on
-    this medium, and the gradient sums read the recomputed binary16 forward fields: measured on
-    MI355X 0.7 % (lambda, alpha, M) to 2 % (rho) and 3.4-3.8 % (mu, rho_f).  The bar, 5e-2, is
-    that measured level with headroom, not an accuracy target: the FP32 batched gradient below
-    is the accurate path (finite differences to 1e-2)."""
-    print("batch16 gradient fp16 vs fp32", flush=True)
+    """The 16-bit batched gradients against the FP32 one.  Binary16 wavefields (the
+    reference's `half` storage) carry ~2e-2 relative trace error on this medium, and the
+    gradient sums read the recomputed binary16 forward fields: measured on MI355X 0.7 %
+    (lambda, alpha, M) to 2 % (rho) and 3.4-3.8 % (mu, rho_f).  Its bar, 5e-2, is that
+    measured level with headroom, not an accuracy target.  FP16T (binary16 + 8-bit residual
+    on tau and p; its checkpoints and segment states carry the residuals and the adjoint
+    passes read them) measured 2.3e-3 (lambda, alpha, M) and 3.2e-3 (rho), held to 5e-3; mu
+    and rho_f 1.3e-2 and 1.5e-2, held to 2e-2: their gradient sums difference the forward
+    velocities and q, which FP16T keeps in plain binary16 (q1 - q0 of the solid update)."""
+    print("batch16 gradient fp16 / fp16t vs fp32", flush=True)
     bt = configs.make_batch(_true(cfg), precision="fp32")
     bt.forward(NT, graph_steps=50)
     obs = [bt.traces(i) for i in range(len(cfg["shots"]))]
     bt.close()
     grads = {}
-    for prec in ("fp32", "fp16"):
+    for prec in ("fp32", "fp16", "fp16t"):
         b = configs.make_batch(cfg, precision=prec)
         grads[prec] = b.gradient(NT, obs, every=50)
         print(f"  {prec}: J {grads[prec][0]:.6e}, {b.last_ms():.0f} ms, {b.gradient_groups()} group(s)", flush=True)
         b.close()
-    (J32, g32), (J16, g16) = grads["fp32"], grads["fp16"]
-    errs = {k: rel_l2(g16[k], g32[k]) for k in g32}
-    print("  fp16 vs fp32 gradient rel L2: " + " ".join(f"{k} {e:.2e}" for k, e in errs.items()))
-    assert all(np.isfinite(g).all() for g in g16.values())
-    assert abs(J16 - J32) <= 0.03 * J32
-    assert max(errs.values()) <= 5e-2
+    J32, g32 = grads["fp32"]
+    errs = {}
+    for prec in ("fp16", "fp16t"):
+        J, g = grads[prec]
+        errs[prec] = {k: rel_l2(g[k], g32[k]) for k in g32}
+        print(f"  {prec} vs fp32 gradient rel L2: " + " ".join(f"{k} {e:.2e}" for k, e in errs[prec].items()),
+              flush=True)
+        assert all(np.isfinite(x).all() for x in g.values())
+        assert abs(J - J32) <= 0.03 * J32
+    assert max(errs["fp16"].values()) <= 5e-2
+    assert max(errs["fp16t"][k] for k in ("lambda", "alpha", "M", "rho")) <= 5e-3
+    assert max(errs["fp16t"][k] for k in ("mu", "rhof")) <= 2e-2
 
 
 def test_batch16_fp32_gradient_matches_finite_differences(cfg):
