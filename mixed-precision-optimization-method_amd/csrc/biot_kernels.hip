@@ -113,7 +113,7 @@ __device__ __forceinline__ float res_inv(int e) { return __builtin_bit_cast(floa
 // FP16M's 24-bit float: the upper three bytes of the float's bits, rounded to nearest even on
 // the dropped byte (overflow rounds to inf).  The upper 16 bits sit in the field's own array
 // (2 bytes per cell, like a binary16 field), the third byte in the cell's 8-byte low-byte
-// record shared by the seven 24-bit fields (byte f - W24_FIRST).  A stencil point of several
+// record shared by the seven 24-bit fields (byte w24_slot(f)).  A stencil point of several
 // of those fields loads the record once (one 8-byte load), and the stress step writes it with
 // one 8-byte store; every access is aligned.
 typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
@@ -131,11 +131,11 @@ __device__ __forceinline__ float w24_join(u32 hi, u32 d, int s)
 template <int F>
 __device__ __forceinline__ float w24_at(const void *hi, const void *rec, u32 idx)
 {
-  constexpr int s = F - W24_FIRST;
+  constexpr int s = w24_slot(F);
   const dw2_t r = at<dw2_t>(rec, idx);
   return w24_join((u32)at<uint16_t>(hi, idx), s < 4 ? r.x : r.y, s & 3);
 }
-// runtime field: lo points at the field's byte of record 0 (record base + f - W24_FIRST)
+// runtime field: lo points at the field's byte of record 0 (record base + w24_slot(f))
 __device__ __forceinline__ float w24_load(const void *hi, const int8_t *lo, u32 idx)
 {
   const u32 b = reinterpret_cast<const uint8_t *>(lo)[idx * (u32)W24_RECORD];
@@ -295,12 +295,8 @@ __device__ __forceinline__ float h2f(uint32_t d, int half)
 {
   return (float)__builtin_bit_cast(_Float16, (uint16_t)(d >> (16 * half)));
 }
-__device__ __forceinline__ void med_rec(const void *base, u32 c, float (&v)[4])
-{
-  typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
-  const dw2_t d = *reinterpret_cast<const dw2_t *>(reinterpret_cast<const char *>(base) + c * 8u);
-  v[0] = h2f(d.x, 0), v[1] = h2f(d.x, 1), v[2] = h2f(d.y, 0), v[3] = h2f(d.y, 1);
-}
+__device__ __forceinline__ dw2_t med_rec(const void *base, u32 c) { return at<dw2_t>(base, c); }
+__device__ __forceinline__ float med_v(const dw2_t &r, int q) { return h2f(q < 2 ? r.x : r.y, q & 1); }
 
 // The velocity half step's 14 coefficients (packed order K_U1X..K_W3Y) of a cell from the
 // FP16M moduli records of the cell (m0) and of its forward neighbours along x, y, z (mn): per
@@ -308,17 +304,17 @@ __device__ __forceinline__ void med_rec(const void *base, u32 c, float (&v)[4])
 // formulas of main.cpp:447-481 with their zero guards (every W when a C1 average is zero,
 // every U when a rho average is).
 template <class AT>
-__device__ __forceinline__ void medium_velocity(const AT &a, const float (&m0)[4], const float (&mn)[3][4],
+__device__ __forceinline__ void medium_velocity(const AT &a, const dw2_t &m0, const dw2_t (&mn)[3],
                                                 float (&kv)[14])
 {
   bool zw = false, zu = false;
   float W3[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const float rho = a.mc[MC_HRHO] * (m0[0] + mn[d][0]);
-    const float rhof = a.mc[MC_HRHOF] * (m0[1] + mn[d][1]);
-    const float c1 = a.mc[MC_HC1] * (m0[2] + mn[d][2]);
-    const float c2 = a.mc[MC_HC2] * (m0[3] + mn[d][3]);
+    const float rho = a.mc[MC_HRHO] * (med_v(m0, 0) + med_v(mn[d], 0));
+    const float rhof = a.mc[MC_HRHOF] * (med_v(m0, 1) + med_v(mn[d], 1));
+    const float c1 = a.mc[MC_HC1] * (med_v(m0, 2) + med_v(mn[d], 2));
+    const float c2 = a.mc[MC_HC2] * (med_v(m0, 3) + med_v(mn[d], 3));
     zw = zw || c1 == 0.0f;
     zu = zu || rho == 0.0f;
     const float rr = __builtin_amdgcn_rcpf(rho);
@@ -473,8 +469,8 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
 
   // FP16M: the cell's moduli record, rotated along z (the forward-z neighbour's record of one
   // plane is the cell's own record of the next)
-  float mown[4] = {0, 0, 0, 0};
-  if constexpr (P::medium) med_rec(a.mv, col + (u32)kb * sz, mown);
+  dw2_t mown = {0u, 0u};
+  if constexpr (P::medium) mown = med_rec(a.mv, col + (u32)kb * sz);
 
   // Every load of a plane is written before the plane's first store.  The compiler cannot
   // prove the arrays distinct, so it never moves a load above a store; in this order all of
@@ -509,11 +505,11 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     const A qx0 = LO<P, FQX, (P::nt_v && NT)>(a, c), qy0 = LO<P, FQY, (P::nt_v && NT)>(a, c), qz0 = LO<P, FQZ, (P::nt_v && NT)>(a, c);
     const A vx0 = LO<P, FVX, (P::nt_v && NT)>(a, c), vy0 = LO<P, FVY, (P::nt_v && NT)>(a, c), vz0 = LO<P, FVZ, (P::nt_v && NT)>(a, c);
     A kv[14];  // K[0..13]: U1 xyz, U2 xyz, W1 xyz, W2 xyz, W3x, W3y
-    float mn[3][4];
+    dw2_t mn[3];
     if constexpr (P::medium) {
-      med_rec(a.mv, c + 1u, mn[0]);
-      med_rec(a.mv, c + sy, mn[1]);
-      med_rec(a.mv, c + sz, mn[2]);
+      mn[0] = med_rec(a.mv, c + 1u);
+      mn[1] = med_rec(a.mv, c + sy);
+      mn[2] = med_rec(a.mv, c + sz);
       medium_velocity(a, mown, mn, kv);
     } else {
       coef_block<P, 14>(a, a.kv, c, 0, kv);
@@ -593,8 +589,7 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
       wp[m] = wp[m + 1];
     }
     if constexpr (P::medium) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) mown[q] = mn[2][q];
+      mown = mn[2];
     }
   }
 }
@@ -732,15 +727,18 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     const A txzn = txz0 + kxz * (uxz + uzx);
     if constexpr (P::w24) {
       // upper halves to the field arrays, the seven low bytes in one record store
-      const u32 r[7] = {w24_round(txxn), w24_round(tyyn), w24_round(tzzn), w24_round(txyn),
-                        w24_round(txzn), w24_round(tyzn), w24_round(pn)};
-      at<uint16_t>(a.f[FTXX], c) = (uint16_t)(r[0] >> 16);
-      at<uint16_t>(a.f[FTYY], c) = (uint16_t)(r[1] >> 16);
-      at<uint16_t>(a.f[FTZZ], c) = (uint16_t)(r[2] >> 16);
-      at<uint16_t>(a.f[FTXY], c) = (uint16_t)(r[3] >> 16);
-      at<uint16_t>(a.f[FTXZ], c) = (uint16_t)(r[4] >> 16);
-      at<uint16_t>(a.f[FTYZ], c) = (uint16_t)(r[5] >> 16);
-      at<uint16_t>(a.f[FP], c) = (uint16_t)(r[6] >> 16);
+      u32 r[7];
+      r[w24_slot(FTXX)] = w24_round(txxn), r[w24_slot(FTYY)] = w24_round(tyyn);
+      r[w24_slot(FTZZ)] = w24_round(tzzn), r[w24_slot(FTXY)] = w24_round(txyn);
+      r[w24_slot(FTXZ)] = w24_round(txzn), r[w24_slot(FTYZ)] = w24_round(tyzn);
+      r[w24_slot(FP)] = w24_round(pn);
+      at<uint16_t>(a.f[FTXX], c) = (uint16_t)(r[w24_slot(FTXX)] >> 16);
+      at<uint16_t>(a.f[FTYY], c) = (uint16_t)(r[w24_slot(FTYY)] >> 16);
+      at<uint16_t>(a.f[FTZZ], c) = (uint16_t)(r[w24_slot(FTZZ)] >> 16);
+      at<uint16_t>(a.f[FTXY], c) = (uint16_t)(r[w24_slot(FTXY)] >> 16);
+      at<uint16_t>(a.f[FTXZ], c) = (uint16_t)(r[w24_slot(FTXZ)] >> 16);
+      at<uint16_t>(a.f[FTYZ], c) = (uint16_t)(r[w24_slot(FTYZ)] >> 16);
+      at<uint16_t>(a.f[FP], c) = (uint16_t)(r[w24_slot(FP)] >> 16);
       at<dw2_t>(a.lo[W24_FIRST], c) = w24_record(r);
     } else {
       ST<P, FP, (P::nt_t && NT)>(a, c, pn);
