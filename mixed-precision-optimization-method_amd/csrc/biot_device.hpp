@@ -27,8 +27,9 @@
 //         stresses' rounding dominates the binary16 error, and the velocity residuals are
 //         a third of FP16C's residual traffic.
 //   FP16M v and q binary16 (scaled); tau and p 24-bit floats, the upper three bytes of a
-//         float (sign, 8 exponent bits, 15 stored significand bits, rounded to nearest even),
-//         packed 3 bytes per cell (one unaligned 4-byte load per value); the medium as two
+//         float (sign, 8 exponent bits, 15 stored significand bits, rounded to nearest even):
+//         the upper two bytes in the field's own array, the third in a per-cell 8-byte record
+//         holding that byte of all seven such fields; the medium as two
 //         per-cell records of binary16 moduli scaled to [-1, 1] (mv:
 //         rho, rho_f, C1, C2 in 8 bytes; mt: C, M, lambda_c, H and the three 4-cell harmonic
 //         means of mu in 16 bytes), from which the kernels form the staggered averages and

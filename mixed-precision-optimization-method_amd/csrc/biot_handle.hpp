@@ -21,7 +21,7 @@ struct biot_handle {
   std::string err;
 
   // device memory
-  void *d_fields = nullptr;          // 13 * n (FP16M: 6 binary16 arrays, then 7 packed 24-bit ones)
+  void *d_fields = nullptr;          // 13 * n elements (FP16M: tau and p as the upper halves of 24-bit floats)
   int8_t *d_lo = nullptr;  // FP16C / FP16T residuals, 13 * n (compensated fields used); FP16M low-byte records, 8 * n
   void *d_coef = nullptr;            // 22 * n (FP16M: the medium records, 18 bytes per cell)
   void *d_kpack = nullptr;           // the coefficients packed per cell (velocity, then stress block)
