@@ -432,7 +432,7 @@ def policy_line(cfg, mode, r, ref_value, steps, n):
             "finite": r["finite"], "tile_bx_by_zchunk": r["shape"]}
 
 
-def perturbed512_section(steps, warmup, modes=("fp32", "fp16m", "fp16")):
+def perturbed512_section(steps, warmup, modes=("fp32", "fp16m", "fp16t", "fp16c", "fp16")):
     """The 512 x 512 x 384 perturbed-medium grid (the FP16-storage workload): throughput of the
     FP32 drop-in and the 16-bit policies over `steps` timed steps after `warmup`."""
     from biot_amd import configs
