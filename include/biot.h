@@ -55,11 +55,12 @@ enum biot_precision {
                      binary16): faster, with a larger error than BIOT_FP16C */
   BIOT_FP16M = 5  /* mixed 16/24-bit storage: v, q binary16 (per-field scaling); tau, p 24-bit
                      floats (the upper three bytes of a float: 8 exponent and 15 stored
-                     significand bits); the medium as 9 binary16 cell moduli (rho, rho_f, C1, C2,
-                     C, M, lambda_c, H, 1/mu) from which the kernels form the staggered averages
-                     and coefficients; float C-PML memory and arithmetic.  Needs the moduli:
-                     biot_set_medium, biot_set_medium_device or biot_set_params (not
-                     biot_set_coefficients); no adjoint or batches. */
+                     significand bits); the medium as per-cell binary16 moduli (rho, rho_f, C1,
+                     C2; C, M, lambda_c, H and the three 4-cell harmonic means of mu) from which
+                     the kernels form the staggered averages and coefficients; float C-PML
+                     memory and arithmetic.  Needs the moduli: biot_set_medium,
+                     biot_set_medium_device or biot_set_params (not biot_set_coefficients); no
+                     adjoint or batches. */
 };
 
 enum biot_field {
@@ -212,8 +213,8 @@ typedef struct biot_params {
 } biot_params;
 int biot_set_params(biot_handle *h, const biot_params *p);
 /* Resets, runs nt steps recording traces (receivers must hold >= nt steps) and keeps the
- * state every `every` steps on the device for the adjoint.  FP32, FP64, FP16 and FP16T handles
- * (FP16T states carry their residuals, and the adjoint reads them).
+ * state every `every` steps on the device for the adjoint.  FP32, FP64, FP16, FP16T and FP16C
+ * handles (FP16T and FP16C states carry their residuals, and the adjoint reads them).
  * every = 1 keeps every step's state (nt + 1 states in one device block, BIOT_ERR_NOMEM if
  * it does not fit): biot_adjoint then reads them instead of recomputing each segment, and
  * gives the same gradients as every = 50 bit for bit. */
@@ -258,8 +259,8 @@ int biot_params_chain(const biot_grid *g, const biot_params *p, const double *co
  * unless a persistent grid is set) covers the tiles of every shot; runs of steps are
  * captured into a HIP graph and replayed.  The gradient sums every shot's parameter
  * gradient into one model-grid volume on the device.  Precisions FP32, FP16 (binary16
- * wavefields, float arithmetic, FP64 gradient sums), FP16T (FP16 with the 8-bit residual on
- * tau and p), FP64. */
+ * wavefields, float arithmetic, FP64 gradient sums), FP16T and FP16C (FP16 with the 8-bit
+ * residual on tau and p, and on v too), FP64. */
 typedef struct biot_batch biot_batch;
 typedef struct biot_batch_config {
   int32_t nx, ny, nz;        /* model grid the windows are cut from (gradient volume) */

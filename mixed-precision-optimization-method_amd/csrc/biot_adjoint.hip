@@ -63,8 +63,9 @@ struct AdjT {
   static constexpr bool scaled = std::is_same<S, _Float16>::value;
   static constexpr bool res = false;
 };
-// FP16T states: binary16 fields and coefficients, an int8 residual on tau and p (value =
-// h + r ulp(h)/256, as the forward kernels store it), float C-PML carries
+// FP16T / FP16C states: binary16 fields and coefficients, an int8 residual on tau and p
+// (FP16C: also on v; value = h + r ulp(h)/256, as the forward kernels store it), float C-PML
+// carries
 struct H16R {};
 template <>
 struct AdjT<H16R> {
@@ -110,7 +111,7 @@ __device__ __forceinline__ const void *fw_carry(const AT &a, int q, int ax)
 {
   return a.slots + (uint64_t)q * a.sb + a.coff[ax];
 }
-// the residuals of field f in state q (FP16T tau and p), or null
+// the residuals of field f in state q (FP16T tau and p, FP16C also v), or null
 template <class AT>
 __device__ __forceinline__ const int8_t *fw_lo(const AT &a, int q, int f)
 {
@@ -740,7 +741,8 @@ hipError_t passes(int prec, int order, F f, int first, int last)
     if (prec == PREC_FP32) e = order == 2 ? f.template run<float, 2>(which) : f.template run<float, 4>(which);
     else if (prec == PREC_FP64) e = order == 2 ? f.template run<double, 2>(which) : f.template run<double, 4>(which);
     else if (prec == PREC_FP16) e = order == 2 ? f.template run<_Float16, 2>(which) : f.template run<_Float16, 4>(which);
-    else if (prec == PREC_FP16T) e = order == 2 ? f.template run<H16R, 2>(which) : f.template run<H16R, 4>(which);
+    else if (prec == PREC_FP16T || prec == PREC_FP16C)
+      e = order == 2 ? f.template run<H16R, 2>(which) : f.template run<H16R, 4>(which);
     else return hipErrorInvalidValue;
     if (e != hipSuccess) return e;
   }
@@ -778,7 +780,7 @@ hipError_t launch_adjoint_inject(int prec, const InjectArgs &a, int it, hipStrea
   if (a.nrec <= 0 || a.ncomp <= 0) return hipSuccess;
   dim3 grid((unsigned)((a.nrec + 255) / 256), (unsigned)a.ncomp);
   if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject<double>, grid, dim3(256), 0, s, a, it);
-  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T)
+  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T || prec == PREC_FP16C)
     hipLaunchKernelGGL(k_adj_inject<float>, grid, dim3(256), 0, s, a, it);
   else return hipErrorInvalidValue;
   return hipGetLastError();
@@ -790,7 +792,7 @@ hipError_t launch_adjoint_inject_batch(int prec, const InjectArgs *tab, int nsho
   if (nshots <= 0 || max_nrec <= 0 || max_ncomp <= 0) return hipSuccess;
   dim3 grid((unsigned)((max_nrec + 255) / 256), (unsigned)max_ncomp, (unsigned)nshots);
   if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject_batch<double>, grid, dim3(256), 0, s, tab, it);
-  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T)
+  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T || prec == PREC_FP16C)
     hipLaunchKernelGGL(k_adj_inject_batch<float>, grid, dim3(256), 0, s, tab, it);
   else return hipErrorInvalidValue;
   return hipGetLastError();

@@ -93,8 +93,8 @@ extern "C" int biot_set_params(biot_handle *h, const biot_params *p)
 extern "C" int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every)
 {
   if (!h) return BIOT_ERR_ARG;
-  if (h->prec == BIOT_FP16C || h->prec == BIOT_FP16M)
-    return afail(h, BIOT_ERR_ARG, "the adjoint path runs on FP32, FP64, FP16 or FP16T handles");
+  if (h->prec == BIOT_FP16M)
+    return afail(h, BIOT_ERR_ARG, "the adjoint path runs on FP32, FP64, FP16, FP16T or FP16C handles");
   if (nt <= 0 || every <= 0) return afail(h, BIOT_ERR_ARG, "nt and every must be positive");
   if (!h->have_coef) return afail(h, BIOT_ERR_STATE, "no medium set");
   if (h->nt_max < nt || h->ncomp == 0) return afail(h, BIOT_ERR_STATE, "receivers must record >= nt steps");

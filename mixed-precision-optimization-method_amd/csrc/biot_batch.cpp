@@ -171,8 +171,8 @@ extern "C" int biot_batch_create(const biot_batch_config *c, biot_batch **out)
     return refuse(nullptr, BIOT_ERR_ARG, "the model grid must be at least 8 cells on every axis");
   if (c->order != 2 && c->order != 4) return refuse(nullptr, BIOT_ERR_ARG, "order must be 2 or 4");
   if (c->precision != BIOT_FP32 && c->precision != BIOT_FP16 && c->precision != BIOT_FP16T &&
-      c->precision != BIOT_FP64)
-    return refuse(nullptr, BIOT_ERR_ARG, "batches run BIOT_FP32, BIOT_FP16, BIOT_FP16T or BIOT_FP64");
+      c->precision != BIOT_FP16C && c->precision != BIOT_FP64)
+    return refuse(nullptr, BIOT_ERR_ARG, "batches run BIOT_FP32, BIOT_FP16, BIOT_FP16T, BIOT_FP16C or BIOT_FP64");
   if (!(c->dx > 0) || !(c->dt > 0) || c->npml < 0)
     return refuse(nullptr, BIOT_ERR_ARG, "dx and dt must be positive and npml >= 0");
   int ndev = 0;
@@ -583,7 +583,7 @@ int gradient_group(biot_batch *b, int s0, int s1, int nt, int every, const float
     BHIP(b, hipMemcpy(b->d_inj, inj.data(), S * sizeof(InjectArgs), hipMemcpyHostToDevice));
   }
   // copy tables: [save ckpt k] nseg, [load ckpt k] nseg, [save slot q] every + 1; 5 pieces per shot
-  // (fields, the three carry blocks, the FP16T residuals: 0 bytes on the other policies)
+  // (fields, the three carry blocks, the FP16T / FP16C residuals: 0 bytes on the other policies)
   constexpr int NP = 5;
   const int per = NP * S;
   auto pieces = [&](biot_handle *h, char *state, bool to_state, CopyEntry *e) {

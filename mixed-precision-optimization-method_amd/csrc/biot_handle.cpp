@@ -151,8 +151,10 @@ void biot_handle::adj_args(AdjArgs &A) const
   A.slots = (const char *)d_slots;
   A.sb = state_bytes();
   A.fbytes = (uint64_t)n * esz;
-  A.loff = prec == BIOT_FP16T ? state_lo_off() : 0;  // FP16T residuals (the adjoint runs no other 16-bit policy)
-  A.lomask = prec == BIOT_FP16T ? comp_fields(prec) : 0u;
+  // FP16T / FP16C residuals (FP16M never reaches the adjoint)
+  const bool res = prec == BIOT_FP16T || prec == BIOT_FP16C;
+  A.loff = res ? state_lo_off() : 0;
+  A.lomask = res ? comp_fields(prec) : 0u;
   for (int ax = 0; ax < 3; ++ax) A.coff[ax] = state_off(1 + ax);
   char *lp = (char *)d_lam;
   for (int f = 0; f < NFIELD; ++f) A.lam[f] = lp + (size_t)f * n * asz;

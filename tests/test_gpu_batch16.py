@@ -78,23 +78,24 @@ def test_batch16_fp16_gradient_against_fp32(cfg):
     (lambda, alpha, M) to 2 % (rho) and 3.4-3.8 % (mu, rho_f).  Its bar, 5e-2, is that
     measured level with headroom, not an accuracy target.  FP16T (binary16 + 8-bit residual
     on tau and p; its checkpoints and segment states carry the residuals and the adjoint
-    passes read them) measured 2.3e-3 (lambda, alpha, M) and 3.2e-3 (rho), held to 5e-3; mu
-    and rho_f 1.3e-2 and 1.5e-2, held to 2e-2: their gradient sums difference the forward
-    velocities and q, which FP16T keeps in plain binary16 (q1 - q0 of the solid update)."""
+    passes read them) measured 2.3e-3 (lambda, alpha, M) and 3.2e-3 (rho), held to 5e-3, and
+    1.3e-2 (mu) and 1.5e-2 (rho_f), held to 2e-2; FP16C (the residual on v as well) measured
+    the same within 10 % (2.2-2.5e-3; 1.28e-2, 1.50e-2), so the velocities' binary16
+    rounding is not what limits mu and rho_f."""
     print("batch16 gradient fp16 / fp16t vs fp32", flush=True)
     bt = configs.make_batch(_true(cfg), precision="fp32")
     bt.forward(NT, graph_steps=50)
     obs = [bt.traces(i) for i in range(len(cfg["shots"]))]
     bt.close()
     grads = {}
-    for prec in ("fp32", "fp16", "fp16t"):
+    for prec in ("fp32", "fp16", "fp16t", "fp16c"):
         b = configs.make_batch(cfg, precision=prec)
         grads[prec] = b.gradient(NT, obs, every=50)
         print(f"  {prec}: J {grads[prec][0]:.6e}, {b.last_ms():.0f} ms, {b.gradient_groups()} group(s)", flush=True)
         b.close()
     J32, g32 = grads["fp32"]
     errs = {}
-    for prec in ("fp16", "fp16t"):
+    for prec in ("fp16", "fp16t", "fp16c"):
         J, g = grads[prec]
         errs[prec] = {k: rel_l2(g[k], g32[k]) for k in g32}
         print(f"  {prec} vs fp32 gradient rel L2: " + " ".join(f"{k} {e:.2e}" for k, e in errs[prec].items()),
@@ -102,8 +103,9 @@ def test_batch16_fp16_gradient_against_fp32(cfg):
         assert all(np.isfinite(x).all() for x in g.values())
         assert abs(J - J32) <= 0.03 * J32
     assert max(errs["fp16"].values()) <= 5e-2
-    assert max(errs["fp16t"][k] for k in ("lambda", "alpha", "M", "rho")) <= 5e-3
-    assert max(errs["fp16t"][k] for k in ("mu", "rhof")) <= 2e-2
+    for prec in ("fp16t", "fp16c"):
+        assert max(errs[prec][k] for k in ("lambda", "alpha", "M", "rho")) <= 5e-3, prec
+        assert max(errs[prec][k] for k in ("mu", "rhof")) <= 2e-2, prec
 
 
 def test_batch16_fp32_gradient_matches_finite_differences(cfg):
