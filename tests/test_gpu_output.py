@@ -28,7 +28,7 @@ def test_streamed_traces_equal_plain_run(chunk):
 
 
 @pytest.mark.parametrize("precision,chunk",
-                         [("fp32", 7), ("fp32", 64), ("fp16", 50), ("fp16c", 500), ("fp16t", 64)])
+                         [("fp32", 7), ("fp32", 64), ("fp16", 50), ("fp16c", 500), ("fp16t", 64), ("fp16m", 64)])
 def test_streamed_snapshots_equal_plain_run(precision, chunk):
     print(f"stream snapshots {precision} chunk {chunk}", flush=True)
     cfg = configs.fwd64(order=4, nt=150)
@@ -89,7 +89,7 @@ def test_text_files_of_a_run(tmp_path):
     s.close()
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp16m"])
 def test_nonfinite_watchdog(precision):
     """A NaN sample in the wavelet (forced from the host; NaN arithmetic does not fault) reaches
     the receiver at the source cell at that step: biot_run reports BIOT_ERR_NONFINITE naming the
