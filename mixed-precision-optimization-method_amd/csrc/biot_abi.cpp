@@ -52,33 +52,18 @@ double half_res_unit(_Float16 h)
   return std::ldexp(1.0, e - 33);
 }
 
-// FP16M's 24-bit float: the upper three bytes of the float, rounded to nearest even on the
-// dropped byte; the upper two in the field array, the third in the cell's low-byte record
-uint32_t w24_bits(float x)
-{
-  uint32_t u;
-  std::memcpy(&u, &x, 4);
-  return u + 0x7Fu + ((u >> 8) & 1u);
-}
-double w24_value(uint16_t hi, uint8_t lo)
-{
-  const uint32_t u = ((uint32_t)hi << 16) | ((uint32_t)lo << 8);
-  float f;
-  std::memcpy(&f, &u, 4);
-  return (double)f;
-}
-
-// host storage-type conversion; lo: the residuals of a compensated field (stride 1) or the
-// low bytes of an FP16M 24-bit field (w24, stride lo_stride), else null
+// host storage-type conversion; lo: the residuals of a compensated field (stride 1) or of an
+// FP16M 24-bit field (w24, stride lo_stride: the byte in the cell's record), else null
 void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst, int8_t *lo,
                 bool w24 = false, size_t lo_stride = 1)
 {
-  if (w24) {
-    uint16_t *d = (uint16_t *)dst;
+  if (w24) {  // binary16 + residual, the residual every lo_stride bytes (the cell's record)
+    _Float16 *d = (_Float16 *)dst;
     for (int64_t c = 0; c < n; ++c) {
-      const uint32_t r = w24_bits((float)(src[c] * scale_inv));
-      d[c] = (uint16_t)(r >> 16);
-      lo[c * lo_stride] = (int8_t)(uint8_t)(r >> 8);
+      const float x = (float)(src[c] * scale_inv);
+      d[c] = (_Float16)x;
+      const double r = std::nearbyint(((double)x - (double)(float)d[c]) / half_res_unit(d[c]));
+      lo[c * lo_stride] = (int8_t)std::min(127.0, std::max(-127.0, r));
     }
     return;
   }
@@ -104,8 +89,9 @@ void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double
                   bool w24 = false, size_t lo_stride = 1)
 {
   if (w24) {
-    const uint16_t *s = (const uint16_t *)src;
-    for (int64_t c = 0; c < n; ++c) dst[c] = w24_value(s[c], (uint8_t)lo[c * lo_stride]) * scale;
+    const _Float16 *s = (const _Float16 *)src;
+    for (int64_t c = 0; c < n; ++c)
+      dst[c] = ((double)(float)s[c] + (double)lo[c * lo_stride] * half_res_unit(s[c])) * scale;
     return;
   }
   if (prec == BIOT_FP32) {

@@ -50,8 +50,8 @@ constexpr uint32_t comp_fields(int prec)
 {
   return prec == PREC_FP16C ? COMP_FIELDS_C : (prec == PREC_FP16T || prec == PREC_FP16M) ? COMP_FIELDS_T : 0u;
 }
-// FP16M: the fields stored as 24-bit floats (txx .. tyz, p): the upper 16 bits in the field's
-// own array, the third byte in the cell's 8-byte low-byte record, at byte w24_slot(f) (the
+// FP16M: the fields stored as 24-bit values (txx .. tyz, p): the binary16 part in the field's
+// own array, the 8-bit residual in the cell's 8-byte low-byte record, at byte w24_slot(f) (the
 // record's last byte is unused)
 constexpr uint32_t W24_FIELDS = COMP_FIELDS_T;
 constexpr int W24_FIRST = 6, W24_RECORD = 8;

@@ -110,49 +110,82 @@ __device__ __forceinline__ int h_exp(_Float16 h)
 __device__ __forceinline__ float res_unit(int e) { return __builtin_bit_cast(float, (u32)(e + 94) << 23); }
 __device__ __forceinline__ float res_inv(int e) { return __builtin_bit_cast(float, (u32)(160 - e) << 23); }
 
-// FP16M's 24-bit float: the upper three bytes of the float's bits, rounded to nearest even on
-// the dropped byte (overflow rounds to inf).  The upper 16 bits sit in the field's own array
-// (2 bytes per cell, like a binary16 field), the third byte in the cell's 8-byte low-byte
-// record shared by the seven 24-bit fields (byte w24_slot(f)).  A stencil point of several
-// of those fields loads the record once (one 8-byte load), and the stress step writes it with
-// one 8-byte store; every access is aligned.
+// FP16M's 24-bit value of tau and p: the binary16 h nearest to the value (on the field's
+// binary16 storage scale) and an 8-bit residual r, value = h + r ulp(h)/256 (the residual form
+// of FP16T, about a 19-bit significand, kept below binary16's normal range too).  h sits in
+// the field's own array (2 bytes per cell, like a binary16 field); r in the cell's 8-byte
+// low-byte record shared by the seven 24-bit fields (byte w24_slot(f)).  A stencil point of
+// several of those fields loads the record once (one 8-byte load), the stress step writes it
+// with one 8-byte store, and every access is aligned.  A stencil tap that needs no more than
+// binary16 precision reads h alone.
 typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ u32 w24_round(float x)
+// ulp(h)/256 from h as a float (exponent below binary16's normal range: the subnormal ulp)
+__device__ __forceinline__ float w24_unit(float hf)
 {
-  const u32 b = __builtin_bit_cast(u32, x);
-  return b + 0x7Fu + ((b >> 8) & 1u);
+  const u32 e = (__builtin_bit_cast(u32, hf) >> 23) & 0xFFu;
+  return __builtin_bit_cast(float, (max(e, 113u) - 18u) << 23);
 }
-// the float of upper half hi (zero-extended) and byte s (0..3) of record dword d
-__device__ __forceinline__ float w24_join(u32 hi, u32 d, int s)
+// h and r of x: r as its byte (two's complement)
+__device__ __forceinline__ void w24_split(float x, uint16_t &hb, u32 &rb)
 {
-  return __builtin_bit_cast(float, __builtin_amdgcn_perm(hi, d, 0x0504000Cu | ((u32)s << 8)));
+  const _Float16 h = (_Float16)x;
+  hb = __builtin_bit_cast(uint16_t, h);
+  const float r = rintf((x - (float)h) * res_inv(h_exp(h)));
+  rb = (u32)(uint8_t)(int8_t)fminf(fmaxf(r, -127.0f), 127.0f);
+}
+// h as a float plus residual byte b
+__device__ __forceinline__ float w24_join(float hf, u32 b)
+{
+  return fmaf((float)(int)(int8_t)(uint8_t)b, w24_unit(hf), hf);
+}
+// byte s (0..6) of a low-byte record
+__device__ __forceinline__ u32 w24_byte(const dw2_t &r, int s)
+{
+  return ((s < 4 ? r.x : r.y) >> (8 * (s & 3))) & 0xFFu;
+}
+// the binary16 part alone, as a float
+__device__ __forceinline__ float w24_hi(const void *hi, u32 idx)
+{
+  return (float)at<_Float16>(hi, idx);
 }
 // value of 24-bit field F at cell idx: hi the field's array, rec the low-byte record array
 template <int F>
 __device__ __forceinline__ float w24_at(const void *hi, const void *rec, u32 idx)
 {
-  constexpr int s = w24_slot(F);
   const dw2_t r = at<dw2_t>(rec, idx);
-  return w24_join((u32)at<uint16_t>(hi, idx), s < 4 ? r.x : r.y, s & 3);
+  return w24_join(w24_hi(hi, idx), w24_byte(r, w24_slot(F)));
+}
+// the same value from a record already loaded
+template <int F>
+__device__ __forceinline__ float w24_rec(const void *hi, const dw2_t &r, u32 idx)
+{
+  return w24_join(w24_hi(hi, idx), w24_byte(r, w24_slot(F)));
+}
+// hf (the binary16 part alone, as a float) completed with field F's residual of record r
+template <int F>
+__device__ __forceinline__ float w24_fill(float hf, const dw2_t &r)
+{
+  return w24_join(hf, w24_byte(r, w24_slot(F)));
 }
 // runtime field: lo points at the field's byte of record 0 (record base + w24_slot(f))
 __device__ __forceinline__ float w24_load(const void *hi, const int8_t *lo, u32 idx)
 {
-  const u32 b = reinterpret_cast<const uint8_t *>(lo)[idx * (u32)W24_RECORD];
-  return __builtin_bit_cast(float, ((u32)at<uint16_t>(hi, idx) << 16) | (b << 8));
+  return w24_join(w24_hi(hi, idx), (u32)(uint8_t)lo[idx * (u32)W24_RECORD]);
 }
 __device__ __forceinline__ void w24_store(void *hi, int8_t *lo, u32 idx, float x)
 {
-  const u32 r = w24_round(x);
-  at<uint16_t>(hi, idx) = (uint16_t)(r >> 16);
-  reinterpret_cast<uint8_t *>(lo)[idx * (u32)W24_RECORD] = (uint8_t)(r >> 8);
+  uint16_t hb;
+  u32 rb;
+  w24_split(x, hb, rb);
+  at<uint16_t>(hi, idx) = hb;
+  reinterpret_cast<uint8_t *>(lo)[idx * (u32)W24_RECORD] = (uint8_t)rb;
 }
-// the low-byte record of seven rounded values (w24_round), r[s] for byte s
-__device__ __forceinline__ dw2_t w24_record(const u32 (&r)[7])
+// the low-byte record of seven residual bytes, b[s] for byte s
+__device__ __forceinline__ dw2_t w24_record(const u32 (&b)[7])
 {
   dw2_t d;
-  d.x = ((r[0] >> 8) & 0xFFu) | (r[1] & 0xFF00u) | ((r[2] << 8) & 0xFF0000u) | ((r[3] << 16) & 0xFF000000u);
-  d.y = ((r[4] >> 8) & 0xFFu) | (r[5] & 0xFF00u) | ((r[6] << 8) & 0xFF0000u);
+  d.x = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+  d.y = b[4] | (b[5] << 8) | (b[6] << 16);
   return d;
 }
 
@@ -225,6 +258,15 @@ __device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
   if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_at<F>(a.f[F], a.lo[W24_FIRST], idx);
   else return load_rt<P>(a, F, idx);
 }
+// the value of field F at a 4th-order stencil's outer tap (c +- 2s, weight 1/24 against the
+// inner taps' 9/8): FP16M's 24-bit fields give their binary16 upper half, which needs no
+// low-byte record; every other field its full value
+template <class P, int F, class AT>
+__device__ __forceinline__ typename P::A LF(const AT &a, u32 idx)
+{
+  if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_hi(a.f[F], idx);
+  else return L<P, F>(a, idx);
+}
 template <class P, int F, bool NT, class AT>
 __device__ __forceinline__ typename P::A LO(const AT &a, u32 idx)
 {
@@ -257,7 +299,7 @@ __device__ __forceinline__ typename P::A dfw(const AT &a, u32 c, u32 s, typename
     return L<P, F>(a, c + s) - fc;
   } else {
     return A(9.0 / 8.0) * (L<P, F>(a, c + s) - fc) +
-           A(-1.0 / 24.0) * (L<P, F>(a, c + 2 * s) - L<P, F>(a, c - s));
+           A(-1.0 / 24.0) * (LF<P, F>(a, c + 2 * s) - L<P, F>(a, c - s));
   }
 }
 // backward difference of field F at c - s/2 (value at c supplied)
@@ -269,7 +311,7 @@ __device__ __forceinline__ typename P::A dbw(const AT &a, u32 c, u32 s, typename
     return fc - L<P, F>(a, c - s);
   } else {
     return A(9.0 / 8.0) * (fc - L<P, F>(a, c - s)) +
-           A(-1.0 / 24.0) * (L<P, F>(a, c + s) - L<P, F>(a, c - 2 * s));
+           A(-1.0 / 24.0) * (L<P, F>(a, c + s) - LF<P, F>(a, c - 2 * s));
   }
 }
 
@@ -458,19 +500,30 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
 
   // z windows: backward-differenced txz, tyz hold planes k-h..k+h-1,
   // forward-differenced tzz, p hold planes k-h+1..k+h.
+  // FP16M at order 4 (zrec): the forward windows' newest plane is an outer tap and holds the
+  // binary16 part alone until the next plane completes it, the first one included
+  constexpr bool zrec = P::w24 && ORDER == 4;
   A wxz[W], wyz[W], wzz[W], wp[W];
 #pragma unroll
   for (int m = 0; m < W - 1; ++m) {
     wxz[m] = L<P, FTXZ>(a, col + (u32)(kb - h + m) * sz);
     wyz[m] = L<P, FTYZ>(a, col + (u32)(kb - h + m) * sz);
-    wzz[m] = L<P, FTZZ>(a, col + (u32)(kb - h + 1 + m) * sz);
-    wp[m] = L<P, FP>(a, col + (u32)(kb - h + 1 + m) * sz);
+    if (zrec && m == W - 2) {
+      wzz[m] = LF<P, FTZZ>(a, col + (u32)(kb - h + 1 + m) * sz);
+      wp[m] = LF<P, FP>(a, col + (u32)(kb - h + 1 + m) * sz);
+    } else {
+      wzz[m] = L<P, FTZZ>(a, col + (u32)(kb - h + 1 + m) * sz);
+      wp[m] = L<P, FP>(a, col + (u32)(kb - h + 1 + m) * sz);
+    }
   }
 
   // FP16M: the cell's moduli record, rotated along z (the forward-z neighbour's record of one
   // plane is the cell's own record of the next)
   dw2_t mown = {0u, 0u};
   if constexpr (P::medium) mown = med_rec(a.mv, col + (u32)kb * sz);
+  // FP16M at order 4: the cell's low-byte record, rotated along z like the moduli record
+  dw2_t rown = {0u, 0u};
+  if constexpr (zrec) rown = at<dw2_t>(a.lo[W24_FIRST], col + (u32)kb * sz);
 
   // Every load of a plane is written before the plane's first store.  The compiler cannot
   // prove the arrays distinct, so it never moves a load above a store; in this order all of
@@ -482,12 +535,31 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     // twelve raw staggered differences (kernel.cu:172-185 without the 1/dx factor)
     A x1, x2, x3, s1, y1, y2, y3, s2, z1, z2, z3, s3;
     {
-      wxz[W - 1] = L<P, FTXZ>(a, c + (u32)(h - 1) * sz);
-      wyz[W - 1] = L<P, FTYZ>(a, c + (u32)(h - 1) * sz);
-      wzz[W - 1] = L<P, FTZZ>(a, c + (u32)h * sz);
-      wp[W - 1] = L<P, FP>(a, c + (u32)h * sz);
+      A txx_c, tyy_c, txy_c;
+      if constexpr (zrec) {
+        // One low-byte record a plane, plane k+1's: it completes the backward windows' new
+        // plane and the forward windows' plane k+1 (read as its binary16 part a plane
+        // earlier), and is the next plane's own record.  The forward windows' new plane k+2
+        // is an outer tap: its binary16 part.
+        const dw2_t r1 = at<dw2_t>(a.lo[W24_FIRST], c + sz);
+        wxz[W - 1] = w24_rec<FTXZ>(a.f[FTXZ], r1, c + sz);
+        wyz[W - 1] = w24_rec<FTYZ>(a.f[FTYZ], r1, c + sz);
+        wzz[W - 2] = w24_fill<FTZZ>(wzz[W - 2], r1);
+        wp[W - 2] = w24_fill<FP>(wp[W - 2], r1);
+        wzz[W - 1] = w24_hi(a.f[FTZZ], c + 2u * sz);
+        wp[W - 1] = w24_hi(a.f[FP], c + 2u * sz);
+        txx_c = w24_rec<FTXX>(a.f[FTXX], rown, c);
+        tyy_c = w24_rec<FTYY>(a.f[FTYY], rown, c);
+        txy_c = w24_rec<FTXY>(a.f[FTXY], rown, c);
+        rown = r1;
+      } else {
+        wxz[W - 1] = L<P, FTXZ>(a, c + (u32)(h - 1) * sz);
+        wyz[W - 1] = L<P, FTYZ>(a, c + (u32)(h - 1) * sz);
+        wzz[W - 1] = L<P, FTZZ>(a, c + (u32)h * sz);
+        wp[W - 1] = L<P, FP>(a, c + (u32)h * sz);
+        txx_c = L<P, FTXX>(a, c), tyy_c = L<P, FTYY>(a, c), txy_c = L<P, FTXY>(a, c);
+      }
       const A txz_c = wxz[h], tyz_c = wyz[h], p_c = wp[h - 1];
-      const A txx_c = L<P, FTXX>(a, c), tyy_c = L<P, FTYY>(a, c), txy_c = L<P, FTXY>(a, c);
       x1 = dfw<P, FTXX, ORDER>(a, c, 1u, txx_c);
       x2 = dbw<P, FTXY, ORDER>(a, c, sy, txy_c);
       x3 = dwin<ORDER>(wxz);
@@ -726,20 +798,24 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     const A tyzn = tyz0 + kyz * (uyz + uzy);
     const A txzn = txz0 + kxz * (uxz + uzx);
     if constexpr (P::w24) {
-      // upper halves to the field arrays, the seven low bytes in one record store
-      u32 r[7];
-      r[w24_slot(FTXX)] = w24_round(txxn), r[w24_slot(FTYY)] = w24_round(tyyn);
-      r[w24_slot(FTZZ)] = w24_round(tzzn), r[w24_slot(FTXY)] = w24_round(txyn);
-      r[w24_slot(FTXZ)] = w24_round(txzn), r[w24_slot(FTYZ)] = w24_round(tyzn);
-      r[w24_slot(FP)] = w24_round(pn);
-      at<uint16_t>(a.f[FTXX], c) = (uint16_t)(r[w24_slot(FTXX)] >> 16);
-      at<uint16_t>(a.f[FTYY], c) = (uint16_t)(r[w24_slot(FTYY)] >> 16);
-      at<uint16_t>(a.f[FTZZ], c) = (uint16_t)(r[w24_slot(FTZZ)] >> 16);
-      at<uint16_t>(a.f[FTXY], c) = (uint16_t)(r[w24_slot(FTXY)] >> 16);
-      at<uint16_t>(a.f[FTXZ], c) = (uint16_t)(r[w24_slot(FTXZ)] >> 16);
-      at<uint16_t>(a.f[FTYZ], c) = (uint16_t)(r[w24_slot(FTYZ)] >> 16);
-      at<uint16_t>(a.f[FP], c) = (uint16_t)(r[w24_slot(FP)] >> 16);
-      at<dw2_t>(a.lo[W24_FIRST], c) = w24_record(r);
+      // binary16 parts to the field arrays, the seven residual bytes in one record store
+      uint16_t hb[7];
+      u32 rb[7];
+      w24_split(txxn, hb[w24_slot(FTXX)], rb[w24_slot(FTXX)]);
+      w24_split(tyyn, hb[w24_slot(FTYY)], rb[w24_slot(FTYY)]);
+      w24_split(tzzn, hb[w24_slot(FTZZ)], rb[w24_slot(FTZZ)]);
+      w24_split(txyn, hb[w24_slot(FTXY)], rb[w24_slot(FTXY)]);
+      w24_split(txzn, hb[w24_slot(FTXZ)], rb[w24_slot(FTXZ)]);
+      w24_split(tyzn, hb[w24_slot(FTYZ)], rb[w24_slot(FTYZ)]);
+      w24_split(pn, hb[w24_slot(FP)], rb[w24_slot(FP)]);
+      at<uint16_t>(a.f[FTXX], c) = hb[w24_slot(FTXX)];
+      at<uint16_t>(a.f[FTYY], c) = hb[w24_slot(FTYY)];
+      at<uint16_t>(a.f[FTZZ], c) = hb[w24_slot(FTZZ)];
+      at<uint16_t>(a.f[FTXY], c) = hb[w24_slot(FTXY)];
+      at<uint16_t>(a.f[FTXZ], c) = hb[w24_slot(FTXZ)];
+      at<uint16_t>(a.f[FTYZ], c) = hb[w24_slot(FTYZ)];
+      at<uint16_t>(a.f[FP], c) = hb[w24_slot(FP)];
+      at<dw2_t>(a.lo[W24_FIRST], c) = w24_record(rb);
     } else {
       ST<P, FP, (P::nt_t && NT)>(a, c, pn);
       ST<P, FTXX, (P::nt_t && NT)>(a, c, txxn);

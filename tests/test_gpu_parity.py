@@ -374,30 +374,40 @@ def test_fp16m_refusals():
 
 
 def _w24_numpy(x32):
-    """24-bit float of a float32 array: round to nearest even on the low byte, keep 3 bytes."""
-    b = x32.view(np.uint32).astype(np.uint64)
-    b = (b + 0x7F + ((b >> 8) & 1)) & 0xFFFFFF00
-    return b.astype(np.uint32).view(np.float32)
+    """FP16M's 24-bit value of a float32 array: the nearest binary16 h and the residual
+    r = rint((x - h) / u) clamped to +-127, u = ulp(h)/256 (2^-32 for subnormal h), value
+    h + r u."""
+    h = x32.astype(np.float16)
+    e = np.maximum((h.view(np.uint16).astype(np.int64) >> 10) & 31, 1)
+    u = np.ldexp(1.0, e - 33)
+    r = np.clip(np.rint((x32.astype(np.float64) - h.astype(np.float64)) / u), -127, 127)
+    return h.astype(np.float64) + r * u
 
 
 def test_fp16m_storage_round_trip():
-    """A written stress field reads back as its 24-bit float (upper three bytes of the float of
-    value / scale, rounded to nearest even) times the scale; the velocity as binary16."""
+    """A written stress field reads back as its 24-bit value (binary16 of value / scale plus an
+    8-bit residual in units of ulp/256) times the scale: within 2^-18 of the value in binary16's
+    normal range, within 2^-32 of the scale below it; the velocity as binary16."""
     n = 24
     s = Solver(n, n, n, 5.0, 4e-4, order=4, precision="fp16m", npml=4)
     s.set_source(s.cell(12, 12, 12), np.ones(8, np.float32), ("p",))
     s.set_medium(media.homogeneous((n, n, n)))
     sc = s.field_scales()
     rng = np.random.default_rng(3)
-    v = rng.standard_normal(n ** 3) * 10.0 ** rng.uniform(-6, 3, n ** 3) * sc["txx"]
-    # floats whose dropped byte is exactly 0x80 (ties), both parities of the kept bits
-    ties = (np.uint32(0x3F800000) + (np.arange(64, dtype=np.uint32) << 8) + np.uint32(0x80)).view(np.float32)
-    v[:64] = ties.astype(np.float64) * sc["txx"]
+    v = rng.standard_normal(n ** 3) * 10.0 ** rng.uniform(-9, 3, n ** 3) * sc["txx"]
+    # values a residual of exactly +-128 units away from their binary16 (clamped to 127)
+    half = (np.float16(1.0) + np.arange(64) * np.float16(2.0 ** -10)).astype(np.float32)
+    v[:64] = (half + np.float32(2.0 ** -11)).astype(np.float64) * sc["txx"]
     s.write_field("txx", v.reshape(n, n, n))
     got = s.field("txx", f64=True).reshape(-1)
     x32 = (v * (1.0 / sc["txx"])).astype(np.float32)
-    want = _w24_numpy(x32).astype(np.float64) * sc["txx"]
+    want = _w24_numpy(x32) * sc["txx"]
     assert np.array_equal(got, want)
+    x = x32.astype(np.float64) * sc["txx"]
+    normal = np.abs(x32) >= 2.0 ** -14
+    assert normal.sum() > n ** 3 // 2 and (~normal).sum() > 100
+    assert np.all(np.abs(got - x)[normal] <= 2.0 ** -18 * np.abs(x)[normal])
+    assert np.all(np.abs(got - x)[~normal] <= 2.0 ** -32 * sc["txx"])
     s.write_field("vx", (v / sc["txx"] * sc["vx"] * 1e-3).reshape(n, n, n))
     gv = s.field("vx", f64=True).reshape(-1)
     wv = (v / sc["txx"] * sc["vx"] * 1e-3 * (1.0 / sc["vx"])).astype(np.float32).astype(np.float16).astype(np.float64) * sc["vx"]
