@@ -258,13 +258,16 @@ __device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
   if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_at<F>(a.f[F], a.lo[W24_FIRST], idx);
   else return load_rt<P>(a, F, idx);
 }
-// the value of field F at a 4th-order stencil's outer tap (c +- 2s, weight 1/24 against the
-// inner taps' 9/8): FP16M's 24-bit fields give their binary16 upper half, which needs no
-// low-byte record; every other field its full value
+// FP16M reads its 24-bit fields at a 4th-order stencil's outer taps (the pair with weight 1/24
+// against the inner pair's 9/8) as their binary16 part, which needs no residual record
+template <class P, int F>
+constexpr bool outer16 = P::w24 && ((W24_FIELDS >> F) & 1u);
+// the value of field F at an outer tap: the binary16 part of FP16M's 24-bit fields, the full
+// value of every other field
 template <class P, int F, class AT>
 __device__ __forceinline__ typename P::A LF(const AT &a, u32 idx)
 {
-  if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_hi(a.f[F], idx);
+  if constexpr (outer16<P, F>) return w24_hi(a.f[F], idx);
   else return L<P, F>(a, idx);
 }
 template <class P, int F, bool NT, class AT>
@@ -298,8 +301,14 @@ __device__ __forceinline__ typename P::A dfw(const AT &a, u32 c, u32 s, typename
   if constexpr (ORDER == 2) {
     return L<P, F>(a, c + s) - fc;
   } else {
-    return A(9.0 / 8.0) * (L<P, F>(a, c + s) - fc) +
-           A(-1.0 / 24.0) * (LF<P, F>(a, c + 2 * s) - L<P, F>(a, c - s));
+    if constexpr (outer16<P, F>) {
+      // FP16M: both outer taps (c + 2s, c - s) as binary16, each an fma on the binary16 value
+      const float o2 = (float)at<_Float16>(a.f[F], c + 2 * s), o1 = (float)at<_Float16>(a.f[F], c - s);
+      return fmaf(A(-1.0 / 24.0), o2, fmaf(A(1.0 / 24.0), o1, A(9.0 / 8.0) * (L<P, F>(a, c + s) - fc)));
+    } else {
+      return A(9.0 / 8.0) * (L<P, F>(a, c + s) - fc) +
+             A(-1.0 / 24.0) * (L<P, F>(a, c + 2 * s) - L<P, F>(a, c - s));
+    }
   }
 }
 // backward difference of field F at c - s/2 (value at c supplied)
@@ -310,8 +319,14 @@ __device__ __forceinline__ typename P::A dbw(const AT &a, u32 c, u32 s, typename
   if constexpr (ORDER == 2) {
     return fc - L<P, F>(a, c - s);
   } else {
-    return A(9.0 / 8.0) * (fc - L<P, F>(a, c - s)) +
-           A(-1.0 / 24.0) * (L<P, F>(a, c + s) - LF<P, F>(a, c - 2 * s));
+    if constexpr (outer16<P, F>) {
+      // FP16M: both outer taps (c + s, c - 2s) as binary16, each an fma on the binary16 value
+      const float o1 = (float)at<_Float16>(a.f[F], c + s), o2 = (float)at<_Float16>(a.f[F], c - 2 * s);
+      return fmaf(A(-1.0 / 24.0), o1, fmaf(A(1.0 / 24.0), o2, A(9.0 / 8.0) * (fc - L<P, F>(a, c - s))));
+    } else {
+      return A(9.0 / 8.0) * (fc - L<P, F>(a, c - s)) +
+             A(-1.0 / 24.0) * (L<P, F>(a, c + s) - L<P, F>(a, c - 2 * s));
+    }
   }
 }
 
