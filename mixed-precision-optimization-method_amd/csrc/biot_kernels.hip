@@ -30,7 +30,8 @@ using u32 = uint32_t;
 // the velocity / stress kernel's own-value loads and field stores are nontemporal (same-box
 // A/B: FP32 velocity kernel -3.5 % time, FP16 velocity kernel -1 % step time; the residual
 // policies 2-3 % slower with them, FP16's stress kernel neutral); medium: the coefficients
-// are formed from the stored moduli (FP16M) instead of read from the packed arrays
+// are formed from the stored moduli (FP16M) instead of read from the packed arrays; outer16:
+// the residual-carrying fields are read at the 4th-order outer taps as binary16
 struct P32 {
   using S = float;
   using SC = float;
@@ -39,6 +40,7 @@ struct P32 {
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = true, nt_t = true;
   static constexpr bool w24 = false, medium = false;
+  static constexpr bool outer16 = false;
 };
 struct P16 {
   using S = _Float16;
@@ -48,6 +50,7 @@ struct P16 {
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = true, nt_t = false;
   static constexpr bool w24 = false, medium = false;
+  static constexpr bool outer16 = false;
 };
 struct P16C {
   using S = _Float16;
@@ -57,6 +60,7 @@ struct P16C {
   static constexpr uint32_t cmask = COMP_FIELDS_C;
   static constexpr bool nt_v = false, nt_t = false;
   static constexpr bool w24 = false, medium = false;
+  static constexpr bool outer16 = false;
 };
 struct P16T {
   using S = _Float16;
@@ -66,6 +70,7 @@ struct P16T {
   static constexpr uint32_t cmask = COMP_FIELDS_T;
   static constexpr bool nt_v = false, nt_t = false;
   static constexpr bool w24 = false, medium = false;
+  static constexpr bool outer16 = true;
 };
 // v, q binary16; tau, p 24-bit floats (upper 16 bits in the field array, the next 8 in the
 // residual array); coefficients from the stored moduli
@@ -77,6 +82,7 @@ struct P16M {
   static constexpr uint32_t cmask = COMP_FIELDS_T;
   static constexpr bool nt_v = false, nt_t = false;
   static constexpr bool w24 = true, medium = true;
+  static constexpr bool outer16 = true;
 };
 struct P64 {
   using S = double;
@@ -86,6 +92,7 @@ struct P64 {
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = false, nt_t = false;
   static constexpr bool w24 = false, medium = false;
+  static constexpr bool outer16 = false;
 };
 
 // element idx of an array, byte offset formed in 32 bits
@@ -258,12 +265,13 @@ __device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
   if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_at<F>(a.f[F], a.lo[W24_FIRST], idx);
   else return load_rt<P>(a, F, idx);
 }
-// FP16M reads its 24-bit fields at a 4th-order stencil's outer taps (the pair with weight 1/24
-// against the inner pair's 9/8) as their binary16 part, which needs no residual record
+// FP16M and FP16T read their tau and p at a 4th-order stencil's outer taps (the pair with
+// weight 1/24 against the inner pair's 9/8) as the binary16 part, without the residual
+// (FP16C reads every tap whole)
 template <class P, int F>
-constexpr bool outer16 = P::w24 && ((W24_FIELDS >> F) & 1u);
-// the value of field F at an outer tap: the binary16 part of FP16M's 24-bit fields, the full
-// value of every other field
+constexpr bool outer16 = P::outer16 && ((P::cmask >> F) & 1u);
+// the value of field F at an outer tap: the binary16 part of FP16M's and FP16T's tau and p,
+// the full value of every other field
 template <class P, int F, class AT>
 __device__ __forceinline__ typename P::A LF(const AT &a, u32 idx)
 {
