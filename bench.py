@@ -349,7 +349,7 @@ def batch_shots(nt=1000, every=50, graph_steps=50):
     obs = [b.traces(i) for i in range(len(cfg["shots"]))]
     b.forward(nt, graph_steps=graph_steps)
     f_ms = b.last_ms()
-    # the same tile shape on persistent grids (blocks striding through the tile list) beside
+    # the same tile shape on persistent grids (blocks taking tiles from a work queue) beside
     # the tuned launch: the measurement behind the autotuner's choice
     persistent = {}
     for blocks in (1024, 2048, 4096):

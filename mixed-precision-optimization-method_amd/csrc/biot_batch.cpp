@@ -47,6 +47,7 @@ struct biot_batch {
   SourceArgs *d_src = nullptr;
   RecordArgs *d_rec = nullptr;
   int *d_it = nullptr;
+  int *d_queue = nullptr;          // 16 ints, the persistent grid's work-queue counters
   int total_tiles = 0;
   int nactive = 0;                 // shots the current tables cover
   int64_t max_nrec = 0;
@@ -147,6 +148,7 @@ biot_batch::~biot_batch()
   free_gradient();
   free_tables();
   dfree(d_it);
+  dfree(d_queue);
   for (auto &s : shots)
     if (s.h) biot_destroy(s.h);
   if (ev0) (void)hipEventDestroy(ev0);
@@ -182,7 +184,9 @@ extern "C" int biot_batch_create(const biot_batch_config *c, biot_batch **out)
   b->cfg = *c;
   if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess ||
       hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-      hipMalloc((void **)&b->d_it, sizeof(int)) != hipSuccess)
+      hipMalloc((void **)&b->d_it, sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&b->d_queue, 16 * sizeof(int)) != hipSuccess ||
+      hipMemset(b->d_queue, 0, 16 * sizeof(int)) != hipSuccess)
     return refuse(b, BIOT_ERR_HIP, "device, stream, event or counter set-up failed");
   *out = b;
   return BIOT_OK;
@@ -336,7 +340,9 @@ int build_tables(biot_batch *b, int s0 = 0, int s1 = -1)
 
 BatchArgs batch_args(const biot_batch *b)
 {
-  return BatchArgs{b->d_step, b->d_tiles, b->nactive, b->total_tiles};
+  // a grid smaller than the tile list takes its tiles from the work queue
+  int *q = b->blocks < b->total_tiles ? b->d_queue : nullptr;
+  return BatchArgs{b->d_step, b->d_tiles, b->nactive, b->total_tiles, q};
 }
 
 // one step of every shot: source, velocity, stress, (record), counter + 1

@@ -182,6 +182,8 @@ struct BatchArgs {
   const StepArgs *shots;    // device, one per shot (gx, gy, gz, zchunk set for the shape)
   const int *tile_start;    // device, nshots + 1 prefix sums of the shots' tile counts
   int nshots, total_tiles;
+  int *queue;               // device, 16 ints (zero between launches): per XCD group the tile
+                            // counter and the count of finished blocks; null: one block per tile
 };
 // blocks: the persistent grid size (a multiple of 8)
 hipError_t launch_step_batch(int prec, int order, bool vel, const BatchArgs &b, LaunchShape sh,

@@ -897,16 +897,45 @@ __device__ __forceinline__ int shot_of(const BatchArgs &b, int t)
 template <class P, int ORDER, bool VEL>
 __global__ void __launch_bounds__(256) k_step_batch(const BatchArgs b)
 {
-  const int x = blockIdx.x & 7, l = blockIdx.x >> 3, per_x = gridDim.x >> 3;
+  // workgroup b goes to XCD b % 8; XCD group x owns the contiguous tile range [begin, end) of
+  // the concatenated tile lists
+  const int x = blockIdx.x & 7;
   const int per = (b.total_tiles + 7) >> 3;
-  const int end = min((x + 1) * per, b.total_tiles);
-  for (int t = x * per + l; t < end; t += per_x) {
+  const int begin = x * per, end = min((x + 1) * per, b.total_tiles);
+  auto tile = [&](int t) {
     const int s = shot_of(b, t);
     const CStepArgs &a = ((CStepArgs *)b.shots)[s];
     const int lt = t - ((CInt *)b.tile_start)[s];
     const int tx = lt % a.gx, r = lt / a.gx, ty = r % a.gy, tz = r / a.gy;
     if (VEL) velocity_tile<P, ORDER, false>(a, tx, ty, tz);
     else stress_tile<P, ORDER, false>(a, tx, ty, tz);
+  };
+  if (!b.queue) {  // fixed mapping: block l of the group takes tiles l, l + per_x, ...
+    const int per_x = gridDim.x >> 3;
+    for (int t = begin + (blockIdx.x >> 3); t < end; t += per_x) tile(t);
+    return;
+  }
+  // Persistent grid with a work queue: a block takes the group's next tile from the group's
+  // counter (one atomic a tile, the next one claimed while the current one runs), so blocks
+  // that drew cheap tiles take more.  Every block ends on one claim past the range; the last
+  // block of the group to make it sets the group's two counters back to zero for the next
+  // launch (no other block of the group touches them after its own last claim).
+  __shared__ int s_next;
+  const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+  int *ctr = b.queue + x, *done = b.queue + 8 + x;
+  if (lead) s_next = atomicAdd(ctr, 1);
+  __syncthreads();
+  int t = begin + s_next;
+  while (t < end) {
+    __syncthreads();  // every thread has read s_next
+    if (lead) s_next = atomicAdd(ctr, 1);
+    tile(t);
+    __syncthreads();  // the next claim is in s_next
+    t = begin + s_next;
+  }
+  if (lead && atomicAdd(done, 1) == (int)(gridDim.x >> 3) - 1) {
+    atomicExch(ctr, 0);
+    atomicExch(done, 0);
   }
 }
 
