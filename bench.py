@@ -45,10 +45,10 @@ WIDTHS = {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 2, 0), "fp16c": (2, 2, 4, 1), "fp
 # 24-bit tau, p (read by the velocity kernel, read and written by the stress kernel)
 RESIDUAL_ACCESSES = {"fp16c": (13, 17, 20), "fp16t": (7, 14, 14), "fp16m": (8, 16, 16)}
 # medium bytes per updated cell where they are not the coefficient arrays: FP16M reads the
-# cell's moduli records (velocity: rho, rho_f, C1, C2 in 8 bytes; stress: C, M, lambda_c, H and
-# the three shear means in 16 bytes; the neighbours' velocity records the averages use are
-# re-reads of the same array)
-MEDIUM_BYTES = {"fp16m": (8, 16, 24)}
+# cell's moduli records (velocity: rho, rho_f, C1, C2 in 8 bytes; stress: C, M, lambda_c, H in
+# 8 bytes and the cell's mu in 2, the shear means formed from the mu of neighbouring cells;
+# the neighbours' records and mu the kernels use are re-reads of the same arrays)
+MEDIUM_BYTES = {"fp16m": (8, 10, 18)}
 
 
 def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):

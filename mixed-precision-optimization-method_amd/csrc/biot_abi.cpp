@@ -227,8 +227,8 @@ extern "C" int biot_create(const biot_grid *g, biot_handle **out)
   const int64_t lim = (int64_t)1 << 32;
   if (h->n * (int64_t)h->esz >= lim)
     return refuse(h, BIOT_ERR_ARG, "grid too large: one field array would exceed 4 GiB");
-  if (h->prec == BIOT_FP16M && h->n * 16 >= lim)
-    return refuse(h, BIOT_ERR_ARG, "grid too large: the FP16M stress medium records (16 bytes a cell) would exceed 4 GiB");
+  if (h->prec == BIOT_FP16M && h->n * 8 >= lim)
+    return refuse(h, BIOT_ERR_ARG, "grid too large: the FP16M medium records (8 bytes a cell) would exceed 4 GiB");
   for (int ax = 0; ax < 3; ++ax)
     if (NCARRY * h->slab[ax] * (int64_t)h->csz >= lim)
       return refuse(h, BIOT_ERR_ARG, "absorbing slabs too large: one axis's C-PML memory would exceed 4 GiB");

@@ -92,8 +92,9 @@ struct StepArgs {
   // in 64 bits: these arrays may span more than 4 GiB)
   const void *kv, *kt;
   float ks[NKC];             // per-array scale (binary16 storage), 1 otherwise
-  // FP16M: the medium records (mv: 4 binary16 values per cell, mt: 8) and constants
-  const void *mv, *mt;
+  // FP16M: the medium records (mv: 4 binary16 values per cell, mt: 4), the cell shear
+  // moduli (mu: binary16) and constants
+  const void *mv, *mt, *mu;
   float mc[NMC];
   // per axis: the C-PML carries (carry type) of the slab cells, in two groups of records — the
   // velocity half step's carries 0..3, then the stress half step's 4..7 — each group slab[ax]

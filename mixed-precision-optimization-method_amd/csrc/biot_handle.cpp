@@ -283,6 +283,7 @@ void biot_handle::fill_args(StepArgs &a) const
     for (int k = 0; k < NKC; ++k) a.k[k] = nullptr;
     a.mv = d_coef;
     a.mt = (const char *)d_coef + mt_offset();
+    a.mu = (const char *)d_coef + mu_offset();
     for (int q = 0; q < NMC; ++q) a.mc[q] = mc[q];
   }
   for (int ax = 0; ax < 3; ++ax) {
@@ -318,7 +319,7 @@ int biot_handle::allocate()
     return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
   };
   // FP16M keeps the medium records (8 + 16 bytes per cell) in place of the coefficient arrays
-  const size_t coef_bytes = prec == BIOT_FP16M ? mt_offset() + (size_t)16 * n : (size_t)NKC * n * esz;
+  const size_t coef_bytes = prec == BIOT_FP16M ? mu_offset() + (size_t)2 * n : (size_t)NKC * n * esz;
   const size_t pack_bytes = prec == BIOT_FP16M ? 0 : (size_t)NKC * n * esz;
   bool ok = alloc(&d_fields, fields_bytes()) && alloc(&d_coef, coef_bytes) &&
             alloc(&d_kpack, pack_bytes) && alloc((void **)&d_flag, 256) && clear_nonfinite() == BIOT_OK;
@@ -673,20 +674,19 @@ int biot_handle::upload_kernel_coefs(const std::vector<double> (&K_in)[NKC])
 }
 
 // FP16M: the medium records from the 9 cell moduli.  Velocity record per cell: rho, rho_f, C1,
-// C2; stress record: C, M, lambda_c, H and the three 4-cell harmonic means of mu of
-// main.cpp:326-337 (zero when a mu of the quad is zero, and on the last plane of each axis,
-// where the reference leaves them zero).  Each stored quantity is scaled by its largest
-// magnitude (the three means by one common scale) and rounded to binary16; the kernels'
-// constants fold the scales back together with the reference's coefficient factors (Cv, Cp,
-// DT, 1/dx as the kernels hold it, the 0.01 x 100 of the stress update) and the field-storage
-// ratios kfac.
+// C2; stress record: C, M, lambda_c, H; and the cell's mu, from which the stress kernel forms
+// the three 4-cell harmonic means of main.cpp:326-337 (zero when a mu of the quad is zero).
+// Each stored quantity is scaled by its largest magnitude and rounded to binary16; the
+// kernels' constants fold the scales back together with the reference's coefficient factors
+// (Cv, Cp, DT, 1/dx as the kernels hold it, the 0.01 x 100 of the stress update) and the
+// field-storage ratios kfac.
 namespace {
 enum { MD_M = 0, MD_HH, MD_H2U, MD_C, MD_C1, MD_C2, MD_RHO, MD_RHOF, MD_MU };
 struct MedCtx {
   const std::vector<float> *mod;
   int nx, ny, nz;
-  double sc[9];        // scales: rho rho_f C1 C2 | C M H2u HH | the shear means
-  uint16_t *mv, *mt;
+  double sc[9];        // scales: rho rho_f C1 C2 | C M H2u HH | mu
+  uint16_t *mv, *mt, *mu;
   double mx[64][9];    // per-thread maxima (first pass)
   std::atomic<int> slot{0};
 };
@@ -696,32 +696,13 @@ inline uint16_t h16(double x)
   return __builtin_bit_cast(uint16_t, h);
 }
 const int MED_SRC[8] = {MD_RHO, MD_RHOF, MD_C1, MD_C2, MD_C, MD_M, MD_H2U, MD_HH};
-// the 4-cell harmonic means of mu at cell c (xy, xz, yz quads), main.cpp:326-337
-void shear_means(const MedCtx *cx, int64_t c, double out[3])
-{
-  const int nx = cx->nx, ny = cx->ny;
-  const int64_t sy = nx, sz = (int64_t)nx * ny;
-  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
-  out[0] = out[1] = out[2] = 0.0;
-  if (i >= nx - 1 || j >= ny - 1 || k >= cx->nz - 1) return;
-  const float *mu = cx->mod[MD_MU].data();
-  auto hm = [&](int64_t a, int64_t b, int64_t d, int64_t e) {
-    if (mu[a] == 0 || mu[b] == 0 || mu[d] == 0 || mu[e] == 0) return 0.0;
-    return 1.0 / (0.25 * (1.0 / mu[a] + 1.0 / mu[b] + 1.0 / mu[d] + 1.0 / mu[e]));
-  };
-  out[0] = hm(c, c + 1, c + sy, c + 1 + sy);
-  out[1] = hm(c, c + 1, c + sz, c + 1 + sz);
-  out[2] = hm(c, c + sy, c + sz, c + sy + sz);
-}
 void med_max_body(int64_t lo, int64_t hi, void *p)
 {
   auto *cx = static_cast<MedCtx *>(p);
   double m[9] = {};
-  double sm[3];
   for (int64_t c = lo; c < hi; ++c) {
     for (int q = 0; q < 8; ++q) m[q] = std::max(m[q], (double)std::fabs(cx->mod[MED_SRC[q]][c]));
-    shear_means(cx, c, sm);
-    m[8] = std::max({m[8], sm[0], sm[1], sm[2]});
+    m[8] = std::max(m[8], (double)std::fabs(cx->mod[MD_MU][c]));
   }
   const int t = cx->slot.fetch_add(1);
   for (int q = 0; q < 9; ++q) cx->mx[t][q] = m[q];
@@ -730,15 +711,12 @@ void med_pack_body(int64_t lo, int64_t hi, void *p)
 {
   auto *cx = static_cast<MedCtx *>(p);
   const std::vector<float> *M = cx->mod;
-  double sm[3];
   for (int64_t c = lo; c < hi; ++c) {
     for (int q = 0; q < 4; ++q) {
       cx->mv[4 * c + q] = h16(M[MED_SRC[q]][c] / cx->sc[q]);
-      cx->mt[8 * c + q] = h16(M[MED_SRC[4 + q]][c] / cx->sc[4 + q]);
+      cx->mt[4 * c + q] = h16(M[MED_SRC[4 + q]][c] / cx->sc[4 + q]);
     }
-    shear_means(cx, c, sm);
-    for (int q = 0; q < 3; ++q) cx->mt[8 * c + 4 + q] = h16(sm[q] / cx->sc[8]);
-    cx->mt[8 * c + 7] = 0;
+    cx->mu[c] = h16(M[MD_MU][c] / cx->sc[8]);
   }
 }
 }  // namespace
@@ -761,12 +739,14 @@ int biot_handle::upload_medium()
     for (int t = 0; t < 64; ++t) sc[q] = std::max(sc[q], cx->mx[t][q]);
     if (!(sc[q] > 0) || !std::isfinite(sc[q])) sc[q] = 1.0;
   }
-  std::vector<uint16_t> mv((size_t)4 * n), mt((size_t)8 * n);
+  std::vector<uint16_t> mv((size_t)4 * n), mt((size_t)4 * n), mu((size_t)n);
   cx->mv = mv.data();
   cx->mt = mt.data();
+  cx->mu = mu.data();
   parallel_for(n, med_pack_body, cx.get());
   if (hipMemcpy(d_coef, mv.data(), mv.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy((char *)d_coef + mt_offset(), mt.data(), mt.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy((char *)d_coef + mt_offset(), mt.data(), mt.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy((char *)d_coef + mu_offset(), mu.data(), mu.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
     err = "medium upload failed";
     return BIOT_ERR_HIP;
   }
@@ -786,6 +766,6 @@ int biot_handle::upload_medium()
   mc[MC_KS1] = (float)(sc[6] * str * kfac[K_S1]);
   mc[MC_KS2] = (float)(sc[7] * str * kfac[K_S2]);
   mc[MC_KS3] = (float)(sc[4] * str * kfac[K_S3]);
-  mc[MC_KSH] = (float)(sc[8] * str * kfac[K_SXY]);
+  mc[MC_KSH] = (float)(4.0 * sc[8] * str * kfac[K_SXY]);  // the kernel forms 1 / sum(1/mu) of a quad
   return BIOT_OK;
 }

@@ -76,6 +76,7 @@ struct biot_handle {
   bool w24(int f) const { return prec == BIOT_FP16M && ((biot::W24_FIELDS >> f) & 1u); }
   size_t fields_bytes() const;
   size_t mt_offset() const { return ((size_t)8 * n + 255) & ~(size_t)255; }  // FP16M stress records
+  size_t mu_offset() const { return (mt_offset() + (size_t)8 * n + 255) & ~(size_t)255; }  // FP16M mu
   bool half() const
   {
     return prec == BIOT_FP16 || prec == BIOT_FP16C || prec == BIOT_FP16T || prec == BIOT_FP16M;

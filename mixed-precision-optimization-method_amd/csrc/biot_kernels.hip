@@ -355,7 +355,7 @@ __device__ __forceinline__ void coef_block(const AT &a, const void *base, u32 c,
   }
 }
 
-// FP16M medium record of cell c: 4 binary16 values in 8 bytes (mv), 8 in 16 bytes (mt)
+// FP16M medium record of cell c: 4 binary16 values in 8 bytes (mv, mt)
 __device__ __forceinline__ float h2f(uint32_t d, int half)
 {
   return (float)__builtin_bit_cast(_Float16, (uint16_t)(d >> (16 * half)));
@@ -400,22 +400,40 @@ __device__ __forceinline__ void medium_velocity(const AT &a, const dw2_t &m0, co
   for (int k = K_U1X; k <= K_U2Z; ++k) kv[k] = zu ? 0.0f : kv[k];
 }
 
-// The stress half step's 8 coefficients (K_P1..K_SYZ) of cell c from its FP16M record: P1, P2,
-// S1, S2, S3 from C, M, lambda_c, H (main.cpp:483-490), the shear ones from the stored
-// 4-cell harmonic means of mu (main.cpp:326-337).
+// FP16M: the reciprocals of the stored mu (scaled binary16) of the 2 x 2 cells (i, j),
+// (i+1, j), (i, j+1), (i+1, j+1) of cell c's plane; a zero mu gives an infinite reciprocal
 template <class AT>
-__device__ __forceinline__ void medium_stress(const AT &a, u32 c, float (&kt)[8])
+__device__ __forceinline__ void mu_quad(const AT &a, u32 c, u32 sy, float (&r)[4])
 {
-  typedef uint32_t dw4_t __attribute__((ext_vector_type(4)));
-  const dw4_t d = *reinterpret_cast<const dw4_t *>(reinterpret_cast<const char *>(a.mt) + c * 16u);
+  r[0] = __builtin_amdgcn_rcpf((float)at<_Float16>(a.mu, c));
+  r[1] = __builtin_amdgcn_rcpf((float)at<_Float16>(a.mu, c + 1u));
+  r[2] = __builtin_amdgcn_rcpf((float)at<_Float16>(a.mu, c + sy));
+  r[3] = __builtin_amdgcn_rcpf((float)at<_Float16>(a.mu, c + 1u + sy));
+}
+
+// The stress half step's 8 coefficients (K_P1..K_SYZ) of cell c: P1, P2, S1, S2, S3 from its
+// FP16M record (C, M, lambda_c, H; main.cpp:483-490), the shear ones from the 4-cell harmonic
+// means of mu of main.cpp:326-337 (xy over the cell's plane, xz and yz over the cell's and the
+// next plane; the constant MC_KSH holds the 4 of 4 / sum(1/mu)).  A zero mu in a quad makes
+// its sum infinite and the mean zero, the reference's guard.  rq holds the quad of the cell's
+// plane and is advanced to the next plane's.
+template <class AT>
+__device__ __forceinline__ void medium_stress(const AT &a, u32 c, u32 sy, u32 sz, float (&rq)[4],
+                                              float (&kt)[8])
+{
+  const dw2_t d = at<dw2_t>(a.mt, c);
   kt[0] = a.mc[MC_KP1] * h2f(d.x, 0);
   kt[1] = a.mc[MC_KP2] * h2f(d.x, 1);
   kt[2] = a.mc[MC_KS1] * h2f(d.y, 0);
   kt[3] = a.mc[MC_KS2] * h2f(d.y, 1);
   kt[4] = a.mc[MC_KS3] * h2f(d.x, 0);
-  kt[5] = a.mc[MC_KSH] * h2f(d.z, 0);
-  kt[6] = a.mc[MC_KSH] * h2f(d.z, 1);
-  kt[7] = a.mc[MC_KSH] * h2f(d.w, 0);
+  float rn[4];
+  mu_quad(a, c + sz, sy, rn);
+  kt[5] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((rq[0] + rq[1]) + (rq[2] + rq[3]));
+  kt[6] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((rq[0] + rq[1]) + (rn[0] + rn[1]));
+  kt[7] = a.mc[MC_KSH] * __builtin_amdgcn_rcpf((rq[0] + rq[2]) + (rn[0] + rn[2]));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) rq[q] = rn[q];
 }
 
 // One folded C-PML update on a carry held in a register: psi = carry + a D;
@@ -735,6 +753,10 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     wvy[m] = L<P, FVY>(a, col + (u32)(kb - h + 1 + m) * sz);
   }
 
+  // FP16M: the mu quad of the first plane, advanced along z by medium_stress
+  float rq[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if constexpr (P::medium) mu_quad(a, col + (u32)kb * sz, sy, rq);
+
   // all of a plane's loads before its first store (see velocity_tile)
   using SC = typename P::SC;
   for (int k = kb; k < ke; ++k) {
@@ -764,7 +786,7 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     const A tzz0 = LO<P, FTZZ, (P::nt_t && NT)>(a, c), txy0 = LO<P, FTXY, (P::nt_t && NT)>(a, c), tyz0 = LO<P, FTYZ, (P::nt_t && NT)>(a, c);
     const A txz0 = LO<P, FTXZ, (P::nt_t && NT)>(a, c);
     A kt[8];  // K[14..21]: P1, P2, S1, S2, S3, Sxy, Sxz, Syz
-    if constexpr (P::medium) medium_stress(a, c, kt);
+    if constexpr (P::medium) medium_stress(a, c, sy, sz, rq, kt);
     else coef_block<P, 8>(a, a.kt, c, K_P1, kt);
     const A s1 = kt[K_S1 - K_P1], s2 = kt[K_S2 - K_P1], s3 = kt[K_S3 - K_P1];
     const A kp1 = kt[0], kp2 = kt[K_P2 - K_P1];

@@ -39,9 +39,9 @@ def test_algorithmic_bytes_closed_form():
     residual_access = {"fp16c": {"v": 10 + 3, "t": 10 + 7, "step": 20},   # v, tau, p carry residuals
                        "fp16t": {"v": 7, "t": 7 + 7, "step": 14},         # tau, p carry residuals
                        "fp16m": {"v": 8, "t": 8 + 8, "step": 16}}         # tau, p low-byte record
-    # FP16M reads binary16 moduli records instead of coefficients: velocity half step rho,
-    # rho_f, C1, C2 (4 x 2 B); stress half step C, M, lambda_c, H and 3 shear means (8 x 2 B)
-    medium_bytes = {"fp16m": {"v": 4 * 2, "t": 8 * 2, "step": 4 * 2 + 8 * 2}}
+    # FP16M reads binary16 moduli instead of coefficients: velocity half step rho, rho_f, C1,
+    # C2 (4 x 2 B); stress half step C, M, lambda_c, H (4 x 2 B) and mu (2 B)
+    medium_bytes = {"fp16m": {"v": 4 * 2, "t": 4 * 2 + 2, "step": 4 * 2 + 4 * 2 + 2}}
     for mode, (wf, wk, wc, wr) in {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 2, 0), "fp16c": (2, 2, 4, 1),
                                    "fp16t": (2, 2, 4, 1), "fp16m": (2, 2, 4, 1), "fp64": (8, 8, 8, 0)}.items():
         v, t, step, u, m = bench.algorithmic_bytes(nx, ny, nz, order, pml, mode)
