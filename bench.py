@@ -36,14 +36,14 @@ def slab_counts(n, L, R, h):
 
 
 # element bytes per storage mode: field, coefficient, C-PML carry, residual (FP16C on the 10
-# compensated fields v, tau, p; FP16T on the 7 of tau, p; FP16M's third byte of its 24-bit
-# tau, p); FP16 keeps every array in binary16, FP16C, FP16T and FP16M keep float carries
+# compensated fields v, tau, p; FP16T on the 7 of tau, p; FP16M's residual record of its tau,
+# p); FP16 keeps every array in binary16, FP16C, FP16T and FP16M keep float carries
 WIDTHS = {"fp32": (4, 4, 4, 0), "fp16": (2, 2, 2, 0), "fp16c": (2, 2, 4, 1), "fp16t": (2, 2, 4, 1),
           "fp16m": (2, 2, 4, 1), "fp64": (8, 8, 8, 0)}
 # residual bytes per updated cell (velocity kernel, stress kernel, one fused step): FP16C and
-# FP16T one per access of a compensated field; FP16M the cell's 8-byte low-byte record of its
-# 24-bit tau, p (read by the velocity kernel, read and written by the stress kernel)
-RESIDUAL_ACCESSES = {"fp16c": (13, 17, 20), "fp16t": (7, 14, 14), "fp16m": (8, 16, 16)}
+# FP16T one per access of a compensated field; FP16M the cell's 4-byte residual record of its
+# tau, p (read by the velocity kernel, read and written by the stress kernel)
+RESIDUAL_ACCESSES = {"fp16c": (13, 17, 20), "fp16t": (7, 14, 14), "fp16m": (4, 8, 8)}
 # medium bytes per updated cell where they are not the coefficient arrays: FP16M reads the
 # cell's moduli records (velocity: rho, rho_f, C1, C2 in 8 bytes; stress: C, M, lambda_c, H in
 # 8 bytes and the cell's mu in 2, the shear means formed from the mu of neighbouring cells;

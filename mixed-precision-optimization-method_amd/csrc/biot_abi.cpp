@@ -43,27 +43,31 @@ int fail(biot_handle *h, int code, const char *fmt, ...)
                   __FILE__, __LINE__);                                                        \
   } while (0)
 
-// binary16 residual unit ulp(h)/256 = 2^(e-33), e the biased exponent (>= 1) of h
-double half_res_unit(_Float16 h)
+// binary16 residual unit ulp(h)/256 = 2^(e-33), e the biased exponent (>= 1) of h; FP16M's
+// 4-bit residuals count ulp(h)/16 = 2^(e-29)
+double half_res_unit(_Float16 h, int bits = 8)
 {
   uint16_t u;
   std::memcpy(&u, &h, 2);
   const int e = std::max((u >> 10) & 31, 1);
-  return std::ldexp(1.0, e - 33);
+  return std::ldexp(1.0, e - 25 - bits);
 }
 
-// host storage-type conversion; lo: the residuals of a compensated field (stride 1) or of an
-// FP16M 24-bit field (w24, stride lo_stride: the byte in the cell's record), else null
+// host storage-type conversion; lo: the residuals of a compensated field (stride 1), or for an
+// FP16M field (w24) the cells' residual records (4 bytes each) and the field's nibble slot,
+// else null
 void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst, int8_t *lo,
-                bool w24 = false, size_t lo_stride = 1)
+                bool w24 = false, int slot = 0)
 {
-  if (w24) {  // binary16 + residual, the residual every lo_stride bytes (the cell's record)
+  if (w24) {  // binary16 + 4-bit residual, patched into nibble `slot` of each cell's record
     _Float16 *d = (_Float16 *)dst;
+    uint32_t *rec = (uint32_t *)lo;
     for (int64_t c = 0; c < n; ++c) {
       const float x = (float)(src[c] * scale_inv);
       d[c] = (_Float16)x;
-      const double r = std::nearbyint(((double)x - (double)(float)d[c]) / half_res_unit(d[c]));
-      lo[c * lo_stride] = (int8_t)std::min(127.0, std::max(-127.0, r));
+      const double r = std::nearbyint(((double)x - (double)(float)d[c]) / half_res_unit(d[c], 4));
+      const uint32_t nib = (uint32_t)(int)std::min(7.0, std::max(-7.0, r)) & 0xFu;
+      rec[c] = (rec[c] & ~(0xFu << (4 * slot))) | (nib << (4 * slot));
     }
     return;
   }
@@ -86,12 +90,15 @@ void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *
   }
 }
 void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double scale, double *dst,
-                  bool w24 = false, size_t lo_stride = 1)
+                  bool w24 = false, int slot = 0)
 {
   if (w24) {
     const _Float16 *s = (const _Float16 *)src;
-    for (int64_t c = 0; c < n; ++c)
-      dst[c] = ((double)(float)s[c] + (double)lo[c * lo_stride] * half_res_unit(s[c])) * scale;
+    const uint32_t *rec = (const uint32_t *)lo;
+    for (int64_t c = 0; c < n; ++c) {
+      const int r = (int)(rec[c] << (28 - 4 * slot)) >> 28;
+      dst[c] = ((double)(float)s[c] + (double)r * half_res_unit(s[c], 4)) * scale;
+    }
     return;
   }
   if (prec == BIOT_FP32) {
@@ -110,8 +117,9 @@ void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double
   }
 }
 
-// bytes spanned by the residual / low bytes of n cells of field f from lo_ptr(f)
-size_t lo_span(const biot_handle *h, int f, int64_t n) { return (size_t)(n - 1) * h->lo_stride(f) + 1; }
+// bytes spanned by the residuals of n cells of field f from lo_copy_ptr(f) (whole records for
+// FP16M)
+size_t lo_span(const biot_handle *h, int f, int64_t n) { return (size_t)n * h->lo_stride(f); }
 
 }  // namespace
 
@@ -845,10 +853,10 @@ extern "C" int biot_read_field_f64(biot_handle *h, int32_t field, double *out)
   HIPCHK(h, hipMemcpy(tmp.data(), h->field_ptr(field), tmp.size(), hipMemcpyDeviceToHost));
   if (h->lo_ptr(field)) {
     lo.resize(lo_span(h, field, h->n));
-    HIPCHK(h, hipMemcpy(lo.data(), h->lo_ptr(field), lo.size(), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(lo.data(), h->lo_copy_ptr(field), lo.size(), hipMemcpyDeviceToHost));
   }
   from_storage(h->prec, tmp.data(), lo.empty() ? nullptr : lo.data(), h->n, h->fscale[field], out, h->w24(field),
-               h->lo_stride(field));
+               w24_slot(field));
   return BIOT_OK;
 }
 
@@ -869,13 +877,13 @@ extern "C" int biot_write_field_f64(biot_handle *h, int32_t field, const double 
   std::vector<unsigned char> tmp((size_t)h->n * h->esz);
   std::vector<int8_t> lo(h->lo_ptr(field) ? lo_span(h, field, h->n) : 0);
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  // the low bytes share their records with the other 24-bit fields: read, patch, write back
-  if (h->w24(field)) HIPCHK(h, hipMemcpy(lo.data(), h->lo_ptr(field), lo.size(), hipMemcpyDeviceToHost));
+  // FP16M's residuals share their records with the other six fields: read, patch, write back
+  if (h->w24(field)) HIPCHK(h, hipMemcpy(lo.data(), h->lo_copy_ptr(field), lo.size(), hipMemcpyDeviceToHost));
   to_storage(h->prec, in, h->n, 1.0 / h->fscale[field], tmp.data(), lo.empty() ? nullptr : lo.data(),
-             h->w24(field), h->lo_stride(field));
+             h->w24(field), w24_slot(field));
   HIPCHK(h, hipMemcpy(h->field_ptr(field), tmp.data(), tmp.size(), hipMemcpyHostToDevice));
   if (!lo.empty())
-    HIPCHK(h, hipMemcpy(h->lo_ptr(field), lo.data(), lo.size(), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->lo_copy_ptr(field), lo.data(), lo.size(), hipMemcpyHostToDevice));
   return BIOT_OK;
 }
 
@@ -894,14 +902,14 @@ extern "C" int biot_snapshot_xz(biot_handle *h, int32_t field, int32_t j, float 
   std::vector<int8_t> lo;
   const size_t ls = h->lo_stride(field);
   if (h->lo_ptr(field)) {
-    // nz rows of nx cells at stride ls (the last cell's byte ends each row)
+    // nz rows of nx cells' residuals (nx bytes, or nx 4-byte records for FP16M)
     lo.resize((size_t)nx * nz * ls);
-    HIPCHK(h, hipMemcpy2D(lo.data(), (size_t)nx * ls, h->lo_ptr(field) + (size_t)j * nx * ls,
+    HIPCHK(h, hipMemcpy2D(lo.data(), (size_t)nx * ls, h->lo_copy_ptr(field) + (size_t)j * nx * ls,
                           (size_t)nx * h->g.ny * ls, lo_span(h, field, nx), (size_t)nz, hipMemcpyDeviceToHost));
   }
   std::vector<double> tmp((size_t)nx * nz);
   from_storage(h->prec, row.data(), lo.empty() ? nullptr : lo.data(), (int64_t)nx * nz,
-               h->fscale[field], tmp.data(), h->w24(field), ls);
+               h->fscale[field], tmp.data(), h->w24(field), w24_slot(field));
   for (size_t c = 0; c < tmp.size(); ++c) out[c] = (float)tmp[c];
   return BIOT_OK;
 }

@@ -50,19 +50,19 @@ constexpr uint32_t comp_fields(int prec)
 {
   return prec == PREC_FP16C ? COMP_FIELDS_C : (prec == PREC_FP16T || prec == PREC_FP16M) ? COMP_FIELDS_T : 0u;
 }
-// FP16M: the fields stored as 24-bit values (txx .. tyz, p): the binary16 part in the field's
-// own array, the 8-bit residual in the cell's 8-byte low-byte record, at byte w24_slot(f) (the
-// record's last byte is unused)
+// FP16M: the fields stored as binary16 + residual (txx .. tyz, p): the binary16 part in the
+// field's own array, the 4-bit residual in the cell's 4-byte residual record, at nibble
+// w24_slot(f) (the record's last nibble is unused)
 constexpr uint32_t W24_FIELDS = COMP_FIELDS_T;
-constexpr int W24_FIRST = 6, W24_RECORD = 8;
-// byte of field f in the record: the four fields the velocity step differences along x
-// (txx, txy, txz, p) in the first dword, tyy, tzz, tyz in the second
+constexpr int W24_FIRST = 6, W24_RECORD = 4;
+// nibble of field f in the record: the four fields the velocity step differences along x
+// (txx, txy, txz, p) in the low half, tyy, tzz, tyz in the high half
 constexpr int w24_slot(int f)
 {
   return f == 6 ? 0 : f == 9 ? 1 : f == 10 ? 2 : f == 12 ? 3 : f == 7 ? 4 : f == 8 ? 5 : 6;
 }
-// the kernels take field W24_FIRST's low-byte pointer as the record base
-static_assert(w24_slot(W24_FIRST) == 0, "field W24_FIRST must own byte 0 of the record");
+// the kernels take field W24_FIRST's residual pointer as the record base
+static_assert(w24_slot(W24_FIRST) == 0, "field W24_FIRST must own nibble 0 of the record");
 // kernel coefficient order
 enum {
   K_U1X = 0, K_U1Y, K_U1Z, K_U2X, K_U2Y, K_U2Z, K_W1X, K_W1Y, K_W1Z, K_W2X, K_W2Y, K_W2Z,

@@ -262,7 +262,7 @@ size_t biot_handle::lo_bytes() const
 int8_t *biot_handle::lo_ptr(int f) const
 {
   if (!d_lo || !((comp_fields(prec) >> f) & 1u)) return nullptr;
-  if (w24(f)) return d_lo + w24_slot(f);  // the field's byte of record 0
+  if (w24(f)) return d_lo + w24_slot(f);  // the record base (256-byte aligned) tagged with the nibble
   return d_lo + (size_t)f * n;
 }
 

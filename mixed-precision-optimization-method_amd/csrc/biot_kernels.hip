@@ -117,83 +117,90 @@ __device__ __forceinline__ int h_exp(_Float16 h)
 __device__ __forceinline__ float res_unit(int e) { return __builtin_bit_cast(float, (u32)(e + 94) << 23); }
 __device__ __forceinline__ float res_inv(int e) { return __builtin_bit_cast(float, (u32)(160 - e) << 23); }
 
-// FP16M's 24-bit value of tau and p: the binary16 h nearest to the value (on the field's
-// binary16 storage scale) and an 8-bit residual r, value = h + r ulp(h)/256 (the residual form
-// of FP16T, about a 19-bit significand, kept below binary16's normal range too).  h sits in
-// the field's own array (2 bytes per cell, like a binary16 field); r in the cell's 8-byte
-// low-byte record shared by the seven 24-bit fields (byte w24_slot(f)).  A stencil point of
-// several of those fields loads the record once (one 8-byte load), the stress step writes it
-// with one 8-byte store, and every access is aligned.  A stencil tap that needs no more than
-// binary16 precision reads h alone.
+// FP16M's tau and p: the binary16 h nearest to the value (on the field's binary16 storage
+// scale) and a 4-bit residual r in -7..7, value = h + r ulp(h)/16 (a 15-bit significand; units
+// of the subnormal ulp below binary16's normal range).  h sits in the field's own array
+// (2 bytes per cell, like a binary16 field); r in the cell's 4-byte residual record shared by
+// the seven fields (nibble w24_slot(f)).  A stencil point of several of those fields loads the
+// record once, the stress step writes it with one 4-byte store, and every access is aligned.
+// A stencil tap that needs no more than binary16 precision reads h alone.  The record array
+// is 256-byte aligned, so a field's residual pointer carries the field's nibble in its low 3
+// bits (the runtime loaders below).
 typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
-// ulp(h)/256 from h as a float (exponent below binary16's normal range: the subnormal ulp)
+// ulp(h)/16 from h as a float (exponent below binary16's normal range: the subnormal ulp)
 __device__ __forceinline__ float w24_unit(float hf)
 {
   const u32 e = (__builtin_bit_cast(u32, hf) >> 23) & 0xFFu;
-  return __builtin_bit_cast(float, (max(e, 113u) - 18u) << 23);
+  return __builtin_bit_cast(float, (max(e, 113u) - 14u) << 23);
 }
-// h and r of x: r as its byte (two's complement)
+// h and r of x: r as its nibble (two's complement)
 __device__ __forceinline__ void w24_split(float x, uint16_t &hb, u32 &rb)
 {
   const _Float16 h = (_Float16)x;
   hb = __builtin_bit_cast(uint16_t, h);
-  const float r = rintf((x - (float)h) * res_inv(h_exp(h)));
-  rb = (u32)(uint8_t)(int8_t)fminf(fmaxf(r, -127.0f), 127.0f);
+  // 2^(29 - e) = 16 / ulp(h), e the biased exponent of h (at least 1)
+  const float inv = __builtin_bit_cast(float, (u32)(156 - h_exp(h)) << 23);
+  const float r = rintf((x - (float)h) * inv);
+  rb = (u32)(int)fminf(fmaxf(r, -7.0f), 7.0f) & 0xFu;
 }
-// h as a float plus residual byte b
-__device__ __forceinline__ float w24_join(float hf, u32 b)
+// residual s (0..6) of a record, sign-extended
+__device__ __forceinline__ int w24_nib(u32 r, int s)
 {
-  return fmaf((float)(int)(int8_t)(uint8_t)b, w24_unit(hf), hf);
+  return (int)(r << (28 - 4 * s)) >> 28;
 }
-// byte s (0..6) of a low-byte record
-__device__ __forceinline__ u32 w24_byte(const dw2_t &r, int s)
+// h as a float plus residual n
+__device__ __forceinline__ float w24_join(float hf, int n)
 {
-  return ((s < 4 ? r.x : r.y) >> (8 * (s & 3))) & 0xFFu;
+  return fmaf((float)n, w24_unit(hf), hf);
 }
 // the binary16 part alone, as a float
 __device__ __forceinline__ float w24_hi(const void *hi, u32 idx)
 {
   return (float)at<_Float16>(hi, idx);
 }
-// value of 24-bit field F at cell idx: hi the field's array, rec the low-byte record array
+// value of field F at cell idx: hi the field's array, rec the residual record array
 template <int F>
 __device__ __forceinline__ float w24_at(const void *hi, const void *rec, u32 idx)
 {
-  const dw2_t r = at<dw2_t>(rec, idx);
-  return w24_join(w24_hi(hi, idx), w24_byte(r, w24_slot(F)));
+  return w24_join(w24_hi(hi, idx), w24_nib(at<u32>(rec, idx), w24_slot(F)));
 }
 // the same value from a record already loaded
 template <int F>
-__device__ __forceinline__ float w24_rec(const void *hi, const dw2_t &r, u32 idx)
+__device__ __forceinline__ float w24_rec(const void *hi, u32 r, u32 idx)
 {
-  return w24_join(w24_hi(hi, idx), w24_byte(r, w24_slot(F)));
+  return w24_join(w24_hi(hi, idx), w24_nib(r, w24_slot(F)));
 }
 // hf (the binary16 part alone, as a float) completed with field F's residual of record r
 template <int F>
-__device__ __forceinline__ float w24_fill(float hf, const dw2_t &r)
+__device__ __forceinline__ float w24_fill(float hf, u32 r)
 {
-  return w24_join(hf, w24_byte(r, w24_slot(F)));
+  return w24_join(hf, w24_nib(r, w24_slot(F)));
 }
-// runtime field: lo points at the field's byte of record 0 (record base + w24_slot(f))
+// runtime field: lo is the record array's base with the field's nibble in its low 3 bits
 __device__ __forceinline__ float w24_load(const void *hi, const int8_t *lo, u32 idx)
 {
-  return w24_join(w24_hi(hi, idx), (u32)(uint8_t)lo[idx * (u32)W24_RECORD]);
+  const uintptr_t t = reinterpret_cast<uintptr_t>(lo);
+  const int s = (int)(t & 7u);
+  return w24_join(w24_hi(hi, idx), w24_nib(at<u32>(reinterpret_cast<const void *>(t - s), idx), s));
 }
+// One field's value: its binary16 part, and its nibble set with two atomics that touch only
+// that nibble's bits (lanes storing other fields of the same cell at once keep theirs).
 __device__ __forceinline__ void w24_store(void *hi, int8_t *lo, u32 idx, float x)
 {
   uint16_t hb;
   u32 rb;
   w24_split(x, hb, rb);
   at<uint16_t>(hi, idx) = hb;
-  reinterpret_cast<uint8_t *>(lo)[idx * (u32)W24_RECORD] = (uint8_t)rb;
+  const uintptr_t t = reinterpret_cast<uintptr_t>(lo);
+  const int s = (int)(t & 7u);
+  u32 *w = &at<u32>(reinterpret_cast<void *>(t - s), idx);
+  atomicAnd(w, ~(0xFu << (4 * s)));
+  atomicOr(w, rb << (4 * s));
 }
-// the low-byte record of seven residual bytes, b[s] for byte s
-__device__ __forceinline__ dw2_t w24_record(const u32 (&b)[7])
+// the residual record of seven nibbles, b[s] for nibble s
+__device__ __forceinline__ u32 w24_record(const u32 (&b)[7])
 {
-  dw2_t d;
-  d.x = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
-  d.y = b[4] | (b[5] << 8) | (b[6] << 16);
-  return d;
+  return b[0] | (b[1] << 4) | (b[2] << 8) | (b[3] << 12) | (b[4] << 16) | (b[5] << 20) | (b[6] << 24);
 }
 
 // value = stored part (+ residual for the compensated fields of the FP16C / FP16T policies;
@@ -257,7 +264,7 @@ __device__ __forceinline__ typename P::A load_rt(const AT &a, int f, u32 idx)
   return load_raw<P>(a.f[f], a.lo[f], (P::cmask >> f) & 1u, idx);
 }
 // compile-time field index: the residual test folds away
-// (FP16M's 24-bit fields read the whole low-byte record, from the base that field W24_FIRST's
+// (FP16M's tau and p read the whole residual record, from the base that field W24_FIRST's
 // pointer holds, so the fields of one stencil point share one record load)
 template <class P, int F, class AT>
 __device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
@@ -562,9 +569,9 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
   // plane is the cell's own record of the next)
   dw2_t mown = {0u, 0u};
   if constexpr (P::medium) mown = med_rec(a.mv, col + (u32)kb * sz);
-  // FP16M at order 4: the cell's low-byte record, rotated along z like the moduli record
-  dw2_t rown = {0u, 0u};
-  if constexpr (zrec) rown = at<dw2_t>(a.lo[W24_FIRST], col + (u32)kb * sz);
+  // FP16M at order 4: the cell's residual record, rotated along z like the moduli record
+  u32 rown = 0u;
+  if constexpr (zrec) rown = at<u32>(a.lo[W24_FIRST], col + (u32)kb * sz);
 
   // Every load of a plane is written before the plane's first store.  The compiler cannot
   // prove the arrays distinct, so it never moves a load above a store; in this order all of
@@ -578,11 +585,11 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
     {
       A txx_c, tyy_c, txy_c;
       if constexpr (zrec) {
-        // One low-byte record a plane, plane k+1's: it completes the backward windows' new
+        // One residual record a plane, plane k+1's: it completes the backward windows' new
         // plane and the forward windows' plane k+1 (read as its binary16 part a plane
         // earlier), and is the next plane's own record.  The forward windows' new plane k+2
         // is an outer tap: its binary16 part.
-        const dw2_t r1 = at<dw2_t>(a.lo[W24_FIRST], c + sz);
+        const u32 r1 = at<u32>(a.lo[W24_FIRST], c + sz);
         wxz[W - 1] = w24_rec<FTXZ>(a.f[FTXZ], r1, c + sz);
         wyz[W - 1] = w24_rec<FTYZ>(a.f[FTYZ], r1, c + sz);
         wzz[W - 2] = w24_fill<FTZZ>(wzz[W - 2], r1);
@@ -843,7 +850,7 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     const A tyzn = tyz0 + kyz * (uyz + uzy);
     const A txzn = txz0 + kxz * (uxz + uzx);
     if constexpr (P::w24) {
-      // binary16 parts to the field arrays, the seven residual bytes in one record store
+      // binary16 parts to the field arrays, the seven residuals in one record store
       uint16_t hb[7];
       u32 rb[7];
       w24_split(txxn, hb[w24_slot(FTXX)], rb[w24_slot(FTXX)]);
@@ -860,7 +867,7 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
       at<uint16_t>(a.f[FTXZ], c) = hb[w24_slot(FTXZ)];
       at<uint16_t>(a.f[FTYZ], c) = hb[w24_slot(FTYZ)];
       at<uint16_t>(a.f[FP], c) = hb[w24_slot(FP)];
-      at<dw2_t>(a.lo[W24_FIRST], c) = w24_record(rb);
+      at<u32>(a.lo[W24_FIRST], c) = w24_record(rb);
     } else {
       ST<P, FP, (P::nt_t && NT)>(a, c, pn);
       ST<P, FTXX, (P::nt_t && NT)>(a, c, txxn);

@@ -38,7 +38,7 @@ def test_algorithmic_bytes_closed_form():
     carry_access = {"v": 8, "t": 8, "step": 16}
     residual_access = {"fp16c": {"v": 10 + 3, "t": 10 + 7, "step": 20},   # v, tau, p carry residuals
                        "fp16t": {"v": 7, "t": 7 + 7, "step": 14},         # tau, p carry residuals
-                       "fp16m": {"v": 8, "t": 8 + 8, "step": 16}}         # tau, p low-byte record
+                       "fp16m": {"v": 4, "t": 4 + 4, "step": 8}}          # tau, p residual record
     # FP16M reads binary16 moduli instead of coefficients: velocity half step rho, rho_f, C1,
     # C2 (4 x 2 B); stress half step C, M, lambda_c, H (4 x 2 B) and mu (2 B)
     medium_bytes = {"fp16m": {"v": 4 * 2, "t": 4 * 2 + 2, "step": 4 * 2 + 4 * 2 + 2}}
