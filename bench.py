@@ -60,8 +60,8 @@ def algorithmic_bytes(nx, ny, nz, order, npml_cells, mode):
     slab.  The fused-step figure is 26 field + 22 coefficient accesses per cell plus
     16 carry accesses per slab membership (one read and one write of 8 carries); FP16C adds
     one residual byte per access of v, tau, p (V: 10 reads + 3 writes, T: 10 + 7, step 20),
-    FP16T per access of tau, p (V: 7 reads, T: 7 + 7, step 14); FP16M reads its 8-byte
-    low-byte record in V and reads and writes it in T (step 16)."""
+    FP16T per access of tau, p (V: 7 reads, T: 7 + 7, step 14); FP16M reads its 4-byte
+    residual record in V and reads and writes it in T (step 8)."""
     wf, wk, wc, wr = WIDTHS[mode]
     rv, rt, rstep = RESIDUAL_ACCESSES.get(mode, (0, 0, 0))
     kv, kt, kstep = MEDIUM_BYTES.get(mode, (14 * wk, 8 * wk, 22 * wk))
@@ -482,6 +482,31 @@ def perturbed512_section(steps, warmup, modes=("fp32", "fp16m", "fp16t", "fp16c"
     return out
 
 
+def perturbed512_accuracy(modes=("fp16m",)):
+    """The 512 x 512 x 384 workload's full 3000 steps: receiver-trace relative L2 error of the
+    16-bit policies against the GPU FP64 path on the 128 x 128 surface receivers (vx, vz, p),
+    the accuracy half of the FP16-storage target (<= 1e-2)."""
+    import numpy as np
+
+    from biot_amd import configs
+
+    cfg = configs.perturbed512()
+    traces = {}
+    for mode in ("fp64",) + tuple(modes):
+        t0 = time.time()
+        s = configs.make_solver(cfg, precision=mode)
+        s.run(cfg["nt"])
+        traces[mode] = s.traces_f64() if mode == "fp64" else s.traces().astype(np.float64)
+        s.close()
+        log(f"[bench] 512x512x384 {mode}: {cfg['nt']} steps for the accuracy check ({time.time() - t0:.0f} s)")
+    ref = traces["fp64"]
+    out = {"steps": cfg["nt"], "reference": "fp64 on the GPU", "receivers": f"{len(cfg['rec'])} surface x (vx, vz, p)"}
+    for mode in modes:
+        errs = [float(np.linalg.norm(traces[mode][c] - ref[c]) / np.linalg.norm(ref[c])) for c in range(3)]
+        out[mode] = {"rel_l2_vx_vz_p": [round(e, 6) for e in errs], "max_rel_l2": round(max(errs), 6)}
+    return out
+
+
 def stream_overlap(cfg, nsteps, chunk=100):
     """Wall time of nsteps with every receiver trace copied to the host: biot_run followed by
     biot_read_traces, against biot_run_stream_traces (copies of each chunk of trace rows on a
@@ -520,6 +545,7 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-512", action="store_true")
     ap.add_argument("--steps-512", type=int, default=300)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the 3000-step FP16M accuracy run of the 512 grid")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
@@ -578,7 +604,17 @@ def main():
             f"step {out[mode]['step_frac']:.3f} of HBM peak")
     if not args.no_512:
         log("[bench] 512x512x384 ...")
-        out["perturbed512"] = perturbed512_section(args.steps_512, args.warmup)
+        sec = perturbed512_section(args.steps_512, args.warmup)
+        if not args.no_accuracy:
+            acc = perturbed512_accuracy()
+            sec["accuracy_3000_steps"] = acc
+            sp, err = sec["fp16m"]["speedup_vs_fp32"], acc["fp16m"]["max_rel_l2"]
+            sec["fp16_storage_target"] = {"target": ">= 1.7x FP32 Mcell-updates/s at <= 1e-2 receiver-trace "
+                                                    "rel L2 against FP64", "policy": "fp16m",
+                                          "speedup_vs_fp32": sp, "max_rel_l2": err,
+                                          "met": bool(sp >= 1.7 and err <= 1e-2)}
+            log(f"[bench] 512x512x384 fp16m: {sp:.2f}x fp32 at {err:.2e} against fp64")
+        out["perturbed512"] = sec
     if not args.no_stream:
         log("[bench] streamed trace output ...")
         out["stream_overlap"] = stream_overlap(cfg, args.steps)

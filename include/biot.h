@@ -53,12 +53,13 @@ enum biot_precision {
                      v, tau and p (value = h + r * ulp(h)/256): compensated half precision */
   BIOT_FP16T = 4, /* as BIOT_FP16C with the residual on tau and p only (velocities plain
                      binary16): faster, with a larger error than BIOT_FP16C */
-  BIOT_FP16M = 5  /* mixed 16/24-bit storage: v, q binary16 (per-field scaling); tau, p 24-bit
-                     (binary16 plus an 8-bit residual in units of ulp/256, the residuals of a
-                     cell packed in one 8-byte record; the velocity step reads the outer taps of
-                     its 4th-order differences as binary16); the medium as per-cell binary16 moduli (rho, rho_f, C1,
-                     C2; C, M, lambda_c, H and the three 4-cell harmonic means of mu) from which
-                     the kernels form the staggered averages and coefficients; float C-PML
+  BIOT_FP16M = 5  /* mixed 16/20-bit storage: v, q binary16 (per-field scaling); tau, p binary16
+                     plus a 4-bit residual in units of ulp/16, the seven residuals of a cell
+                     packed in one 4-byte record (the velocity step reads the outer taps of its
+                     4th-order differences as binary16); the medium as per-cell binary16
+                     moduli (rho, rho_f, C1, C2; C, M, lambda_c, H; mu) from which the kernels
+                     form the staggered averages, the harmonic means of mu and the
+                     coefficients; float C-PML
                      memory and arithmetic.  Needs the moduli: biot_set_medium,
                      biot_set_medium_device or biot_set_params (not biot_set_coefficients); no
                      adjoint or batches. */

@@ -26,14 +26,13 @@
 //   FP16T as FP16C with the residual on tau and p only (the velocities plain binary16): the
 //         stresses' rounding dominates the binary16 error, and the velocity residuals are
 //         a third of FP16C's residual traffic.
-//   FP16M v and q binary16 (scaled); tau and p 24-bit floats, the upper three bytes of a
-//         float (sign, 8 exponent bits, 15 stored significand bits, rounded to nearest even):
-//         the upper two bytes in the field's own array, the third in a per-cell 8-byte record
-//         holding that byte of all seven such fields; the medium as two
-//         per-cell records of binary16 moduli scaled to [-1, 1] (mv:
-//         rho, rho_f, C1, C2 in 8 bytes; mt: C, M, lambda_c, H and the three 4-cell harmonic
-//         means of mu in 16 bytes), from which the kernels form the staggered averages and
-//         the 22 coefficients in registers; float C-PML carries.
+//   FP16M v and q binary16 (scaled); tau and p binary16 (scaled) plus a 4-bit residual in
+//         units of ulp/16, the binary16 part in the field's own array, the residuals of the
+//         seven such fields in one 4-byte record per cell; the medium as per-cell binary16
+//         moduli scaled to [-1, 1] (mv: rho, rho_f, C1, C2 in 8 bytes; mt: C, M, lambda_c, H
+//         in 8 bytes; mu: 2 bytes), from which the kernels form the staggered averages, the
+//         4-cell harmonic means of mu and the 22 coefficients in registers; float C-PML
+//         carries.
 //   FP64  everything double.
 #pragma once
 #include <hip/hip_runtime.h>

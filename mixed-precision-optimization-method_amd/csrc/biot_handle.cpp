@@ -12,7 +12,7 @@
 using namespace biot;
 
 // One time-stepping state (wavefields + C-PML carries) in one device buffer: the fields
-// block at offset 0, the residual / low-byte block of the 16-bit policies that have one, then
+// block at offset 0, the residual block of the 16-bit policies that have one, then
 // the three carry blocks, each piece starting on a 256-byte boundary.
 static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 size_t biot_handle::state_lo_off() const { return round256((size_t)NFIELD * n * esz); }

@@ -21,8 +21,8 @@ struct biot_handle {
   std::string err;
 
   // device memory
-  void *d_fields = nullptr;          // 13 * n elements (FP16M: tau and p as the upper halves of 24-bit floats)
-  int8_t *d_lo = nullptr;  // FP16C / FP16T residuals, 13 * n (compensated fields used); FP16M low-byte records, 8 * n
+  void *d_fields = nullptr;          // 13 * n elements (FP16M: tau and p as their binary16 parts)
+  int8_t *d_lo = nullptr;  // FP16C / FP16T residuals, 13 * n (compensated fields used); FP16M residual records, 4 * n
   void *d_coef = nullptr;            // 22 * n (FP16M: the medium records, 18 bytes per cell)
   void *d_kpack = nullptr;           // the coefficients packed per cell (velocity, then stress block)
   void *d_carry[3] = {nullptr, nullptr, nullptr};
@@ -110,7 +110,7 @@ struct biot_handle {
   float *d_adjsrc = nullptr;                 // adjoint source traces
   size_t state_bytes() const;
   size_t state_off(int piece) const;  // 0 fields, 1..3 carries of x, y, z, 4 total
-  size_t state_lo_off() const;        // the residual / low-byte block (lo_bytes())
+  size_t state_lo_off() const;        // the residual block (lo_bytes())
   void save_state(void *dst);
   void load_state(const void *src);
   void free_adjoint();

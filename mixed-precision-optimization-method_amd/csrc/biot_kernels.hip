@@ -26,7 +26,7 @@ namespace {
 
 using u32 = uint32_t;
 
-// cmask: the fields that carry a residual (w24: the 24-bit float form instead); nt_v / nt_t:
+// cmask: the fields that carry a residual (w24: in FP16M's packed record); nt_v / nt_t:
 // the velocity / stress kernel's own-value loads and field stores are nontemporal (same-box
 // A/B: FP32 velocity kernel -3.5 % time, FP16 velocity kernel -1 % step time; the residual
 // policies 2-3 % slower with them, FP16's stress kernel neutral); medium: the coefficients
@@ -72,8 +72,8 @@ struct P16T {
   static constexpr bool w24 = false, medium = false;
   static constexpr bool outer16 = true;
 };
-// v, q binary16; tau, p 24-bit floats (upper 16 bits in the field array, the next 8 in the
-// residual array); coefficients from the stored moduli
+// v, q binary16; tau, p binary16 + 4-bit residual (the binary16 part in the field array, the
+// residual in the cell's record); coefficients from the stored moduli
 struct P16M {
   using S = _Float16;
   using SC = float;
