@@ -54,12 +54,12 @@ double half_res_unit(_Float16 h, int bits = 8)
 }
 
 // host storage-type conversion; lo: the residuals of a compensated field (stride 1), or for an
-// FP16M field (w24) the cells' residual records (4 bytes each) and the field's nibble slot,
+// FP16M field (packed) the cells' residual records (4 bytes each) and the field's nibble slot,
 // else null
 void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *dst, int8_t *lo,
-                bool w24 = false, int slot = 0)
+                bool packed = false, int slot = 0)
 {
-  if (w24) {  // binary16 + 4-bit residual, patched into nibble `slot` of each cell's record
+  if (packed) {  // binary16 + 4-bit residual, patched into nibble `slot` of each cell's record
     _Float16 *d = (_Float16 *)dst;
     uint32_t *rec = (uint32_t *)lo;
     for (int64_t c = 0; c < n; ++c) {
@@ -90,9 +90,9 @@ void to_storage(int prec, const double *src, int64_t n, double scale_inv, void *
   }
 }
 void from_storage(int prec, const void *src, const int8_t *lo, int64_t n, double scale, double *dst,
-                  bool w24 = false, int slot = 0)
+                  bool packed = false, int slot = 0)
 {
-  if (w24) {
+  if (packed) {
     const _Float16 *s = (const _Float16 *)src;
     const uint32_t *rec = (const uint32_t *)lo;
     for (int64_t c = 0; c < n; ++c) {
@@ -855,8 +855,8 @@ extern "C" int biot_read_field_f64(biot_handle *h, int32_t field, double *out)
     lo.resize(lo_span(h, field, h->n));
     HIPCHK(h, hipMemcpy(lo.data(), h->lo_copy_ptr(field), lo.size(), hipMemcpyDeviceToHost));
   }
-  from_storage(h->prec, tmp.data(), lo.empty() ? nullptr : lo.data(), h->n, h->fscale[field], out, h->w24(field),
-               w24_slot(field));
+  from_storage(h->prec, tmp.data(), lo.empty() ? nullptr : lo.data(), h->n, h->fscale[field], out, h->packed(field),
+               pk_slot(field));
   return BIOT_OK;
 }
 
@@ -878,9 +878,9 @@ extern "C" int biot_write_field_f64(biot_handle *h, int32_t field, const double 
   std::vector<int8_t> lo(h->lo_ptr(field) ? lo_span(h, field, h->n) : 0);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   // FP16M's residuals share their records with the other six fields: read, patch, write back
-  if (h->w24(field)) HIPCHK(h, hipMemcpy(lo.data(), h->lo_copy_ptr(field), lo.size(), hipMemcpyDeviceToHost));
+  if (h->packed(field)) HIPCHK(h, hipMemcpy(lo.data(), h->lo_copy_ptr(field), lo.size(), hipMemcpyDeviceToHost));
   to_storage(h->prec, in, h->n, 1.0 / h->fscale[field], tmp.data(), lo.empty() ? nullptr : lo.data(),
-             h->w24(field), w24_slot(field));
+             h->packed(field), pk_slot(field));
   HIPCHK(h, hipMemcpy(h->field_ptr(field), tmp.data(), tmp.size(), hipMemcpyHostToDevice));
   if (!lo.empty())
     HIPCHK(h, hipMemcpy(h->lo_copy_ptr(field), lo.data(), lo.size(), hipMemcpyHostToDevice));
@@ -909,7 +909,7 @@ extern "C" int biot_snapshot_xz(biot_handle *h, int32_t field, int32_t j, float 
   }
   std::vector<double> tmp((size_t)nx * nz);
   from_storage(h->prec, row.data(), lo.empty() ? nullptr : lo.data(), (int64_t)nx * nz,
-               h->fscale[field], tmp.data(), h->w24(field), w24_slot(field));
+               h->fscale[field], tmp.data(), h->packed(field), pk_slot(field));
   for (size_t c = 0; c < tmp.size(); ++c) out[c] = (float)tmp[c];
   return BIOT_OK;
 }

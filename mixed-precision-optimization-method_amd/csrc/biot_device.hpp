@@ -51,17 +51,17 @@ constexpr uint32_t comp_fields(int prec)
 }
 // FP16M: the fields stored as binary16 + residual (txx .. tyz, p): the binary16 part in the
 // field's own array, the 4-bit residual in the cell's 4-byte residual record, at nibble
-// w24_slot(f) (the record's last nibble is unused)
-constexpr uint32_t W24_FIELDS = COMP_FIELDS_T;
-constexpr int W24_FIRST = 6, W24_RECORD = 4;
+// pk_slot(f) (the record's last nibble is unused)
+constexpr uint32_t PK_FIELDS = COMP_FIELDS_T;
+constexpr int PK_FIRST = 6, PK_RECORD = 4;
 // nibble of field f in the record: the four fields the velocity step differences along x
 // (txx, txy, txz, p) in the low half, tyy, tzz, tyz in the high half
-constexpr int w24_slot(int f)
+constexpr int pk_slot(int f)
 {
   return f == 6 ? 0 : f == 9 ? 1 : f == 10 ? 2 : f == 12 ? 3 : f == 7 ? 4 : f == 8 ? 5 : 6;
 }
-// the kernels take field W24_FIRST's residual pointer as the record base
-static_assert(w24_slot(W24_FIRST) == 0, "field W24_FIRST must own nibble 0 of the record");
+// the kernels take field PK_FIRST's residual pointer as the record base
+static_assert(pk_slot(PK_FIRST) == 0, "field PK_FIRST must own nibble 0 of the record");
 // kernel coefficient order
 enum {
   K_U1X = 0, K_U1Y, K_U1Z, K_U2X, K_U2Y, K_U2Z, K_W1X, K_W1Y, K_W1Z, K_W2X, K_W2Y, K_W2Z,
@@ -198,8 +198,8 @@ hipError_t launch_set_counter(int *it, int v, hipStream_t s);
 // multiply a stored array by f (lo: its residuals or null); carries use prec of the carry
 // type, i.e. pass PREC_FP32 for FP16C/FP16T carries and PREC_FP16 for FP16 carries
 hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
-// the same for an FP16M 24-bit array of n values
-hipError_t launch_rescale24(void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
+// the same for an FP16M tau or p array of n values (lo: its tagged residual pointer)
+hipError_t launch_rescale_pk(void *p, int8_t *lo, int64_t n, double f, hipStream_t s);
 // element e of every 4-element record times f[e] (carry groups whose carries scale differently)
 hipError_t launch_rescale4(int prec, void *p, int64_t nrec, const double f[4], hipStream_t s);
 // the xz slice y = j of one field as float values (stored value times scale): out[k*nx + i]

@@ -256,13 +256,13 @@ void *biot_handle::field_ptr(int f) const { return (char *)d_fields + (size_t)f 
 size_t biot_handle::lo_bytes() const
 {
   if (!comp_fields(prec)) return 0;
-  return prec == BIOT_FP16M ? (size_t)W24_RECORD * n : (size_t)NFIELD * n;
+  return prec == BIOT_FP16M ? (size_t)PK_RECORD * n : (size_t)NFIELD * n;
 }
 
 int8_t *biot_handle::lo_ptr(int f) const
 {
   if (!d_lo || !((comp_fields(prec) >> f) & 1u)) return nullptr;
-  if (w24(f)) return d_lo + w24_slot(f);  // the record base (256-byte aligned) tagged with the nibble
+  if (packed(f)) return d_lo + pk_slot(f);  // the record base (256-byte aligned) tagged with the nibble
   return d_lo + (size_t)f * n;
 }
 
@@ -514,7 +514,7 @@ int biot_handle::rescale_state(const float old_scale[NFIELD])
   bool ok = true;
   for (int f = 0; f < NFIELD && ok; ++f) {
     const double r = (double)old_scale[f] / (double)fscale[f];
-    if (w24(f)) ok = r == 1.0 || launch_rescale24(field_ptr(f), lo_ptr(f), n, r, stream) == hipSuccess;
+    if (packed(f)) ok = r == 1.0 || launch_rescale_pk(field_ptr(f), lo_ptr(f), n, r, stream) == hipSuccess;
     else ok = rescale(prec, field_ptr(f), lo_ptr(f), n, r);
   }
   const double rt = (double)old_scale[6] / fscale[6], rv = (double)old_scale[0] / fscale[0];

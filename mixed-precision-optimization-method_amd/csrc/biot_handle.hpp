@@ -67,16 +67,16 @@ struct biot_handle {
   ~biot_handle();
   void *field_ptr(int f) const;
   // residual array of field f (FP16C / FP16T, stride 1), or for FP16M's tau and p the base of
-  // the residual records (stride W24_RECORD) plus the field's nibble (the kernels' runtime
+  // the residual records (stride PK_RECORD) plus the field's nibble (the kernels' runtime
   // loaders read it from the pointer's low 3 bits), or null
   int8_t *lo_ptr(int f) const;
   // where a host copy of field f's residuals starts: FP16M's record base, else lo_ptr(f)
-  int8_t *lo_copy_ptr(int f) const { return w24(f) ? d_lo : lo_ptr(f); }
-  size_t lo_stride(int f) const { return w24(f) ? biot::W24_RECORD : 1; }
+  int8_t *lo_copy_ptr(int f) const { return packed(f) ? d_lo : lo_ptr(f); }
+  size_t lo_stride(int f) const { return packed(f) ? biot::PK_RECORD : 1; }
   size_t lo_bytes() const;
   // FP16M keeps tau and p as binary16 + a 4-bit residual: the binary16 part in the field array,
   // the residual in a per-cell record shared by the seven fields
-  bool w24(int f) const { return prec == BIOT_FP16M && ((biot::W24_FIELDS >> f) & 1u); }
+  bool packed(int f) const { return prec == BIOT_FP16M && ((biot::PK_FIELDS >> f) & 1u); }
   size_t fields_bytes() const;
   size_t mt_offset() const { return ((size_t)8 * n + 255) & ~(size_t)255; }  // FP16M stress records
   size_t mu_offset() const { return (mt_offset() + (size_t)8 * n + 255) & ~(size_t)255; }  // FP16M mu

@@ -26,7 +26,7 @@ namespace {
 
 using u32 = uint32_t;
 
-// cmask: the fields that carry a residual (w24: in FP16M's packed record); nt_v / nt_t:
+// cmask: the fields that carry a residual (packed: in FP16M's packed record); nt_v / nt_t:
 // the velocity / stress kernel's own-value loads and field stores are nontemporal (same-box
 // A/B: FP32 velocity kernel -3.5 % time, FP16 velocity kernel -1 % step time; the residual
 // policies 2-3 % slower with them, FP16's stress kernel neutral); medium: the coefficients
@@ -39,7 +39,7 @@ struct P32 {
   static constexpr bool comp = false, scaled = false;
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = true, nt_t = true;
-  static constexpr bool w24 = false, medium = false;
+  static constexpr bool packed = false, medium = false;
   static constexpr bool outer16 = false;
 };
 struct P16 {
@@ -49,7 +49,7 @@ struct P16 {
   static constexpr bool comp = false, scaled = true;
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = true, nt_t = false;
-  static constexpr bool w24 = false, medium = false;
+  static constexpr bool packed = false, medium = false;
   static constexpr bool outer16 = false;
 };
 struct P16C {
@@ -59,7 +59,7 @@ struct P16C {
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_C;
   static constexpr bool nt_v = false, nt_t = false;
-  static constexpr bool w24 = false, medium = false;
+  static constexpr bool packed = false, medium = false;
   static constexpr bool outer16 = false;
 };
 struct P16T {
@@ -69,7 +69,7 @@ struct P16T {
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_T;
   static constexpr bool nt_v = false, nt_t = false;
-  static constexpr bool w24 = false, medium = false;
+  static constexpr bool packed = false, medium = false;
   static constexpr bool outer16 = true;
 };
 // v, q binary16; tau, p binary16 + 4-bit residual (the binary16 part in the field array, the
@@ -81,7 +81,7 @@ struct P16M {
   static constexpr bool comp = true, scaled = true;
   static constexpr uint32_t cmask = COMP_FIELDS_T;
   static constexpr bool nt_v = false, nt_t = false;
-  static constexpr bool w24 = true, medium = true;
+  static constexpr bool packed = true, medium = true;
   static constexpr bool outer16 = true;
 };
 struct P64 {
@@ -91,7 +91,7 @@ struct P64 {
   static constexpr bool comp = false, scaled = false;
   static constexpr uint32_t cmask = 0u;
   static constexpr bool nt_v = false, nt_t = false;
-  static constexpr bool w24 = false, medium = false;
+  static constexpr bool packed = false, medium = false;
   static constexpr bool outer16 = false;
 };
 
@@ -121,20 +121,20 @@ __device__ __forceinline__ float res_inv(int e) { return __builtin_bit_cast(floa
 // scale) and a 4-bit residual r in -7..7, value = h + r ulp(h)/16 (a 15-bit significand; units
 // of the subnormal ulp below binary16's normal range).  h sits in the field's own array
 // (2 bytes per cell, like a binary16 field); r in the cell's 4-byte residual record shared by
-// the seven fields (nibble w24_slot(f)).  A stencil point of several of those fields loads the
+// the seven fields (nibble pk_slot(f)).  A stencil point of several of those fields loads the
 // record once, the stress step writes it with one 4-byte store, and every access is aligned.
 // A stencil tap that needs no more than binary16 precision reads h alone.  The record array
 // is 256-byte aligned, so a field's residual pointer carries the field's nibble in its low 3
 // bits (the runtime loaders below).
 typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
 // ulp(h)/16 from h as a float (exponent below binary16's normal range: the subnormal ulp)
-__device__ __forceinline__ float w24_unit(float hf)
+__device__ __forceinline__ float pk_unit(float hf)
 {
   const u32 e = (__builtin_bit_cast(u32, hf) >> 23) & 0xFFu;
   return __builtin_bit_cast(float, (max(e, 113u) - 14u) << 23);
 }
 // h and r of x: r as its nibble (two's complement)
-__device__ __forceinline__ void w24_split(float x, uint16_t &hb, u32 &rb)
+__device__ __forceinline__ void pk_split(float x, uint16_t &hb, u32 &rb)
 {
   const _Float16 h = (_Float16)x;
   hb = __builtin_bit_cast(uint16_t, h);
@@ -144,52 +144,52 @@ __device__ __forceinline__ void w24_split(float x, uint16_t &hb, u32 &rb)
   rb = (u32)(int)fminf(fmaxf(r, -7.0f), 7.0f) & 0xFu;
 }
 // residual s (0..6) of a record, sign-extended
-__device__ __forceinline__ int w24_nib(u32 r, int s)
+__device__ __forceinline__ int pk_nib(u32 r, int s)
 {
   return (int)(r << (28 - 4 * s)) >> 28;
 }
 // h as a float plus residual n
-__device__ __forceinline__ float w24_join(float hf, int n)
+__device__ __forceinline__ float pk_join(float hf, int n)
 {
-  return fmaf((float)n, w24_unit(hf), hf);
+  return fmaf((float)n, pk_unit(hf), hf);
 }
 // the binary16 part alone, as a float
-__device__ __forceinline__ float w24_hi(const void *hi, u32 idx)
+__device__ __forceinline__ float pk_hi(const void *hi, u32 idx)
 {
   return (float)at<_Float16>(hi, idx);
 }
 // value of field F at cell idx: hi the field's array, rec the residual record array
 template <int F>
-__device__ __forceinline__ float w24_at(const void *hi, const void *rec, u32 idx)
+__device__ __forceinline__ float pk_at(const void *hi, const void *rec, u32 idx)
 {
-  return w24_join(w24_hi(hi, idx), w24_nib(at<u32>(rec, idx), w24_slot(F)));
+  return pk_join(pk_hi(hi, idx), pk_nib(at<u32>(rec, idx), pk_slot(F)));
 }
 // the same value from a record already loaded
 template <int F>
-__device__ __forceinline__ float w24_rec(const void *hi, u32 r, u32 idx)
+__device__ __forceinline__ float pk_rec(const void *hi, u32 r, u32 idx)
 {
-  return w24_join(w24_hi(hi, idx), w24_nib(r, w24_slot(F)));
+  return pk_join(pk_hi(hi, idx), pk_nib(r, pk_slot(F)));
 }
 // hf (the binary16 part alone, as a float) completed with field F's residual of record r
 template <int F>
-__device__ __forceinline__ float w24_fill(float hf, u32 r)
+__device__ __forceinline__ float pk_fill(float hf, u32 r)
 {
-  return w24_join(hf, w24_nib(r, w24_slot(F)));
+  return pk_join(hf, pk_nib(r, pk_slot(F)));
 }
 // runtime field: lo is the record array's base with the field's nibble in its low 3 bits
-__device__ __forceinline__ float w24_load(const void *hi, const int8_t *lo, u32 idx)
+__device__ __forceinline__ float pk_load(const void *hi, const int8_t *lo, u32 idx)
 {
   const uintptr_t t = reinterpret_cast<uintptr_t>(lo);
   const int s = (int)(t & 7u);
-  return w24_join(w24_hi(hi, idx), w24_nib(at<u32>(reinterpret_cast<const void *>(t - s), idx), s));
+  return pk_join(pk_hi(hi, idx), pk_nib(at<u32>(reinterpret_cast<const void *>(t - s), idx), s));
 }
 // One field's value: its binary16 part, and its nibble set with two atomics that touch only
 // that nibble's bits (lanes storing other fields of the same cell at once keep theirs).
-__device__ __forceinline__ void w24_store(void *hi, int8_t *lo, u32 idx, float x)
+__device__ __forceinline__ void pk_store(void *hi, int8_t *lo, u32 idx, float x)
 {
   uint16_t hb;
   u32 rb;
-  w24_split(x, hb, rb);
+  pk_split(x, hb, rb);
   at<uint16_t>(hi, idx) = hb;
   const uintptr_t t = reinterpret_cast<uintptr_t>(lo);
   const int s = (int)(t & 7u);
@@ -198,24 +198,24 @@ __device__ __forceinline__ void w24_store(void *hi, int8_t *lo, u32 idx, float x
   atomicOr(w, rb << (4 * s));
 }
 // the residual record of seven nibbles, b[s] for nibble s
-__device__ __forceinline__ u32 w24_record(const u32 (&b)[7])
+__device__ __forceinline__ u32 pk_record(const u32 (&b)[7])
 {
   return b[0] | (b[1] << 4) | (b[2] << 8) | (b[3] << 12) | (b[4] << 16) | (b[5] << 20) | (b[6] << 24);
 }
 
 // value = stored part (+ residual for the compensated fields of the FP16C / FP16T policies;
-// the 24-bit form for FP16M's tau and p)
+// FP16M's tau and p with the residual of their packed record)
 template <class P>
 __device__ __forceinline__ typename P::A load_raw(const void *f, const int8_t *lo, bool comp_f,
                                                   u32 idx)
 {
   using S = typename P::S;
-  if constexpr (P::w24) {
-    if (comp_f) return w24_load(f, lo, idx);
+  if constexpr (P::packed) {
+    if (comp_f) return pk_load(f, lo, idx);
   }
   const S h = at<S>(f, idx);
   typename P::A v = (typename P::A)h;
-  if constexpr (P::comp && !P::w24) {
+  if constexpr (P::comp && !P::packed) {
     if (comp_f) v += (float)at<int8_t>(lo, idx) * res_unit(h_exp(h));
   }
   return v;
@@ -225,14 +225,14 @@ template <class P, bool NT>
 __device__ __forceinline__ typename P::A load_own(const void *f, const int8_t *lo, bool comp_f, u32 idx)
 {
   using S = typename P::S;
-  if constexpr (P::w24) {
-    if (comp_f) return w24_load(f, lo, idx);
+  if constexpr (P::packed) {
+    if (comp_f) return pk_load(f, lo, idx);
   }
   S h;
   if constexpr (NT) h = __builtin_nontemporal_load(&at<S>(f, idx));
   else h = at<S>(f, idx);
   typename P::A v = (typename P::A)h;
-  if constexpr (P::comp && !P::w24) {
+  if constexpr (P::comp && !P::packed) {
     if (comp_f) v += (float)at<int8_t>(lo, idx) * res_unit(h_exp(h));
   }
   return v;
@@ -242,16 +242,16 @@ __device__ __forceinline__ void store_raw(void *f, int8_t *lo, bool comp_f, u32 
                                           typename P::A x)
 {
   using S = typename P::S;
-  if constexpr (P::w24) {
+  if constexpr (P::packed) {
     if (comp_f) {
-      w24_store(f, lo, idx, x);
+      pk_store(f, lo, idx, x);
       return;
     }
   }
   const S h = (S)x;
   if constexpr (NT) __builtin_nontemporal_store(h, &at<S>(f, idx));
   else at<S>(f, idx) = h;
-  if constexpr (P::comp && !P::w24) {
+  if constexpr (P::comp && !P::packed) {
     if (comp_f) {
       const float r = rintf((x - (float)h) * res_inv(h_exp(h)));
       at<int8_t>(lo, idx) = (int8_t)fminf(fmaxf(r, -127.0f), 127.0f);
@@ -264,12 +264,12 @@ __device__ __forceinline__ typename P::A load_rt(const AT &a, int f, u32 idx)
   return load_raw<P>(a.f[f], a.lo[f], (P::cmask >> f) & 1u, idx);
 }
 // compile-time field index: the residual test folds away
-// (FP16M's tau and p read the whole residual record, from the base that field W24_FIRST's
+// (FP16M's tau and p read the whole residual record, from the base that field PK_FIRST's
 // pointer holds, so the fields of one stencil point share one record load)
 template <class P, int F, class AT>
 __device__ __forceinline__ typename P::A L(const AT &a, u32 idx)
 {
-  if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_at<F>(a.f[F], a.lo[W24_FIRST], idx);
+  if constexpr (P::packed && ((PK_FIELDS >> F) & 1u)) return pk_at<F>(a.f[F], a.lo[PK_FIRST], idx);
   else return load_rt<P>(a, F, idx);
 }
 // FP16M and FP16T read their tau and p at a 4th-order stencil's outer taps (the pair with
@@ -282,13 +282,13 @@ constexpr bool outer16 = P::outer16 && ((P::cmask >> F) & 1u);
 template <class P, int F, class AT>
 __device__ __forceinline__ typename P::A LF(const AT &a, u32 idx)
 {
-  if constexpr (outer16<P, F>) return w24_hi(a.f[F], idx);
+  if constexpr (outer16<P, F>) return pk_hi(a.f[F], idx);
   else return L<P, F>(a, idx);
 }
 template <class P, int F, bool NT, class AT>
 __device__ __forceinline__ typename P::A LO(const AT &a, u32 idx)
 {
-  if constexpr (P::w24 && ((W24_FIELDS >> F) & 1u)) return w24_at<F>(a.f[F], a.lo[W24_FIRST], idx);
+  if constexpr (P::packed && ((PK_FIELDS >> F) & 1u)) return pk_at<F>(a.f[F], a.lo[PK_FIRST], idx);
   else return load_own<P, NT>(a.f[F], a.lo[F], (P::cmask >> F) & 1u, idx);
 }
 template <class P, int F, bool NT, class AT>
@@ -550,7 +550,7 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
   // forward-differenced tzz, p hold planes k-h+1..k+h.
   // FP16M at order 4 (zrec): the forward windows' newest plane is an outer tap and holds the
   // binary16 part alone until the next plane completes it, the first one included
-  constexpr bool zrec = P::w24 && ORDER == 4;
+  constexpr bool zrec = P::packed && ORDER == 4;
   A wxz[W], wyz[W], wzz[W], wp[W];
 #pragma unroll
   for (int m = 0; m < W - 1; ++m) {
@@ -571,7 +571,7 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
   if constexpr (P::medium) mown = med_rec(a.mv, col + (u32)kb * sz);
   // FP16M at order 4: the cell's residual record, rotated along z like the moduli record
   u32 rown = 0u;
-  if constexpr (zrec) rown = at<u32>(a.lo[W24_FIRST], col + (u32)kb * sz);
+  if constexpr (zrec) rown = at<u32>(a.lo[PK_FIRST], col + (u32)kb * sz);
 
   // Every load of a plane is written before the plane's first store.  The compiler cannot
   // prove the arrays distinct, so it never moves a load above a store; in this order all of
@@ -589,16 +589,16 @@ __device__ __forceinline__ void velocity_tile(const AT &a, int tx, int ty, int t
         // plane and the forward windows' plane k+1 (read as its binary16 part a plane
         // earlier), and is the next plane's own record.  The forward windows' new plane k+2
         // is an outer tap: its binary16 part.
-        const u32 r1 = at<u32>(a.lo[W24_FIRST], c + sz);
-        wxz[W - 1] = w24_rec<FTXZ>(a.f[FTXZ], r1, c + sz);
-        wyz[W - 1] = w24_rec<FTYZ>(a.f[FTYZ], r1, c + sz);
-        wzz[W - 2] = w24_fill<FTZZ>(wzz[W - 2], r1);
-        wp[W - 2] = w24_fill<FP>(wp[W - 2], r1);
-        wzz[W - 1] = w24_hi(a.f[FTZZ], c + 2u * sz);
-        wp[W - 1] = w24_hi(a.f[FP], c + 2u * sz);
-        txx_c = w24_rec<FTXX>(a.f[FTXX], rown, c);
-        tyy_c = w24_rec<FTYY>(a.f[FTYY], rown, c);
-        txy_c = w24_rec<FTXY>(a.f[FTXY], rown, c);
+        const u32 r1 = at<u32>(a.lo[PK_FIRST], c + sz);
+        wxz[W - 1] = pk_rec<FTXZ>(a.f[FTXZ], r1, c + sz);
+        wyz[W - 1] = pk_rec<FTYZ>(a.f[FTYZ], r1, c + sz);
+        wzz[W - 2] = pk_fill<FTZZ>(wzz[W - 2], r1);
+        wp[W - 2] = pk_fill<FP>(wp[W - 2], r1);
+        wzz[W - 1] = pk_hi(a.f[FTZZ], c + 2u * sz);
+        wp[W - 1] = pk_hi(a.f[FP], c + 2u * sz);
+        txx_c = pk_rec<FTXX>(a.f[FTXX], rown, c);
+        tyy_c = pk_rec<FTYY>(a.f[FTYY], rown, c);
+        txy_c = pk_rec<FTXY>(a.f[FTXY], rown, c);
         rown = r1;
       } else {
         wxz[W - 1] = L<P, FTXZ>(a, c + (u32)(h - 1) * sz);
@@ -849,25 +849,25 @@ __device__ __forceinline__ void stress_tile(const AT &a, int tx, int ty, int tz)
     const A txyn = txy0 + kxy * (uxy + uyx);
     const A tyzn = tyz0 + kyz * (uyz + uzy);
     const A txzn = txz0 + kxz * (uxz + uzx);
-    if constexpr (P::w24) {
+    if constexpr (P::packed) {
       // binary16 parts to the field arrays, the seven residuals in one record store
       uint16_t hb[7];
       u32 rb[7];
-      w24_split(txxn, hb[w24_slot(FTXX)], rb[w24_slot(FTXX)]);
-      w24_split(tyyn, hb[w24_slot(FTYY)], rb[w24_slot(FTYY)]);
-      w24_split(tzzn, hb[w24_slot(FTZZ)], rb[w24_slot(FTZZ)]);
-      w24_split(txyn, hb[w24_slot(FTXY)], rb[w24_slot(FTXY)]);
-      w24_split(txzn, hb[w24_slot(FTXZ)], rb[w24_slot(FTXZ)]);
-      w24_split(tyzn, hb[w24_slot(FTYZ)], rb[w24_slot(FTYZ)]);
-      w24_split(pn, hb[w24_slot(FP)], rb[w24_slot(FP)]);
-      at<uint16_t>(a.f[FTXX], c) = hb[w24_slot(FTXX)];
-      at<uint16_t>(a.f[FTYY], c) = hb[w24_slot(FTYY)];
-      at<uint16_t>(a.f[FTZZ], c) = hb[w24_slot(FTZZ)];
-      at<uint16_t>(a.f[FTXY], c) = hb[w24_slot(FTXY)];
-      at<uint16_t>(a.f[FTXZ], c) = hb[w24_slot(FTXZ)];
-      at<uint16_t>(a.f[FTYZ], c) = hb[w24_slot(FTYZ)];
-      at<uint16_t>(a.f[FP], c) = hb[w24_slot(FP)];
-      at<u32>(a.lo[W24_FIRST], c) = w24_record(rb);
+      pk_split(txxn, hb[pk_slot(FTXX)], rb[pk_slot(FTXX)]);
+      pk_split(tyyn, hb[pk_slot(FTYY)], rb[pk_slot(FTYY)]);
+      pk_split(tzzn, hb[pk_slot(FTZZ)], rb[pk_slot(FTZZ)]);
+      pk_split(txyn, hb[pk_slot(FTXY)], rb[pk_slot(FTXY)]);
+      pk_split(txzn, hb[pk_slot(FTXZ)], rb[pk_slot(FTXZ)]);
+      pk_split(tyzn, hb[pk_slot(FTYZ)], rb[pk_slot(FTYZ)]);
+      pk_split(pn, hb[pk_slot(FP)], rb[pk_slot(FP)]);
+      at<uint16_t>(a.f[FTXX], c) = hb[pk_slot(FTXX)];
+      at<uint16_t>(a.f[FTYY], c) = hb[pk_slot(FTYY)];
+      at<uint16_t>(a.f[FTZZ], c) = hb[pk_slot(FTZZ)];
+      at<uint16_t>(a.f[FTXY], c) = hb[pk_slot(FTXY)];
+      at<uint16_t>(a.f[FTXZ], c) = hb[pk_slot(FTXZ)];
+      at<uint16_t>(a.f[FTYZ], c) = hb[pk_slot(FTYZ)];
+      at<uint16_t>(a.f[FP], c) = hb[pk_slot(FP)];
+      at<u32>(a.lo[PK_FIRST], c) = pk_record(rb);
     } else {
       ST<P, FP, (P::nt_t && NT)>(a, c, pn);
       ST<P, FTXX, (P::nt_t && NT)>(a, c, txxn);
@@ -1036,12 +1036,12 @@ __global__ void k_set_counter(int *itp, int v)
   if (threadIdx.x == 0) atomicExch(itp, v);
 }
 
-// In-place multiply of an FP16M 24-bit field (lo: its byte of record 0).
-__global__ void k_rescale24(void *p, int8_t *lo, int64_t n, double f)
+// In-place multiply of an FP16M tau or p field (lo: its tagged residual pointer).
+__global__ void k_rescale_pk(void *p, int8_t *lo, int64_t n, double f)
 {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    w24_store(p, lo, (u32)i, w24_load(p, lo, (u32)i) * (float)f);
+    pk_store(p, lo, (u32)i, pk_load(p, lo, (u32)i) * (float)f);
 }
 
 // In-place multiply of a stored array (re-expresses values when a storage scale changes).
@@ -1270,7 +1270,7 @@ hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hi
     case PREC_FP16T:
       hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, lo, n, f);
       break;
-    case PREC_FP16M:  // the binary16 arrays (v, q); launch_rescale24 takes tau and p
+    case PREC_FP16M:  // the binary16 arrays (v, q); launch_rescale_pk takes tau and p
       hipLaunchKernelGGL(k_rescale<_Float16>, dim3(blocks), dim3(256), 0, s, (_Float16 *)p, lo, n, f);
       break;
     case PREC_FP64:
@@ -1281,11 +1281,11 @@ hipError_t launch_rescale(int prec, void *p, int8_t *lo, int64_t n, double f, hi
   return hipGetLastError();
 }
 
-hipError_t launch_rescale24(void *p, int8_t *lo, int64_t n, double f, hipStream_t s)
+hipError_t launch_rescale_pk(void *p, int8_t *lo, int64_t n, double f, hipStream_t s)
 {
   if (n <= 0 || !p || !lo) return hipSuccess;
   const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_rescale24, dim3(blocks), dim3(256), 0, s, p, lo, n, f);
+  hipLaunchKernelGGL(k_rescale_pk, dim3(blocks), dim3(256), 0, s, p, lo, n, f);
   return hipGetLastError();
 }
 

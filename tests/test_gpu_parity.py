@@ -87,7 +87,7 @@ def test_fp64_fwd64_matches_oracle_f64(order):
 def test_half_precision_fwd64_vs_fp64_oracle(precision, order, bar):
     """Receiver traces of the 16-bit storage modes against the FP64 oracle, 500 steps, the
     workloads' order 4 and the reference's order 2.  fp16c (binary16 + 8-bit residual on v,
-    tau, p), fp16t (residual on tau, p) and fp16m (24-bit tau, p; binary16 v, q and moduli)
+    tau, p), fp16t (residual on tau, p) and fp16m (tau, p binary16 + a packed 4-bit residual; binary16 v, q and moduli)
     must meet the 1e-2 target; plain binary16 storage (every array binary16, as the
     reference's `half` arrays) misses it: 2.5e-2 is its measured level (1.6e-2) with
     headroom, not the target."""
@@ -131,7 +131,7 @@ def test_half_precision_full_size_perturbed512():
     sets in one run: the 128 x 128 surface receivers (vx, vz, p) and the reference's own layout,
     the vertical line through the source column (kernel.cu:2027-2031; the source cell left
     out) recording txx.  FP16C (binary16 + 8-bit residual on v, tau, p) and FP16T (residual on
-    tau, p) and FP16M (24-bit tau, p; binary16 v, q and moduli) within 1e-2 of the GPU FP64
+    tau, p) and FP16M (tau, p binary16 + a packed 4-bit residual; binary16 v, q and moduli) within 1e-2 of the GPU FP64
     path on both; plain FP16 is reported (it misses 1e-2: 2.0e-2 measured)."""
     import time
 
@@ -373,7 +373,7 @@ def test_fp16m_refusals():
     s.close()
 
 
-def _w24_numpy(x32):
+def _fp16m_numpy(x32):
     """FP16M's stored value of a float32 array: the nearest binary16 h and the 4-bit residual
     r = rint((x - h) / u) clamped to +-7, u = ulp(h)/16 (2^-28 for subnormal h), value
     h + r u."""
@@ -405,7 +405,7 @@ def test_fp16m_storage_round_trip():
     assert np.array_equal(s.field("tyy", f64=True), tyy)
     got = s.field("txx", f64=True).reshape(-1)
     x32 = (v * (1.0 / sc["txx"])).astype(np.float32)
-    want = _w24_numpy(x32) * sc["txx"]
+    want = _fp16m_numpy(x32) * sc["txx"]
     assert np.array_equal(got, want)
     x = x32.astype(np.float64) * sc["txx"]
     normal = np.abs(x32) >= 2.0 ** -14
