@@ -217,8 +217,9 @@ def adjoint_shot(repeats=2, fd_cells=2):
     state modes: every step's state kept (1001 states, 140 GB: the sweep recomputes nothing;
     the headline figures) and 50-step checkpoints with segment recomputation (`segments`).
     Observed traces come from the FP64 run of the true model.  The sweep's roofline uses
-    adjoint_bytes; the FP32 gradient is checked against central differences of the FP64
-    forward misfit (step 1e-3 of the value) at `fd_cells` seeded cells for lambda and rho."""
+    adjoint_bytes; the FP32 gradient is checked against fourth-order central differences of the
+    FP64 forward misfit (step 1e-3 of the value) at `fd_cells` seeded cells for lambda and
+    rho."""
     import math
 
     import numpy as np
@@ -296,14 +297,18 @@ def adjoint_shot(repeats=2, fd_cells=2):
         for name in ("lambda", "rho"):
             hstep = 1e-3 * abs(p0[name][k, j, i])
             d = {}
-            for m in (1, -1):
+            for m in (1, -1, 2, -2):
                 p = {kk: v.copy() for kk, v in p0.items()}
                 p[name][k, j, i] += m * hstep
                 s64.set_params(p)
                 s64.reset()
                 s64.run(nt)
                 d[m] = s64.traces_f64()
-            fd = math.fsum(((d[1] - d[-1]) * (0.5 * (d[1] + d[-1]) - obs64)).ravel()) / (2 * hstep)
+
+            def dJ(a, b):  # J(a) - J(b) from the double traces, without the rounding of J itself
+                return math.fsum(((d[a] - d[b]) * (0.5 * (d[a] + d[b]) - obs64)).ravel())
+
+            fd = (8 * dJ(1, -1) - dJ(2, -2)) / (12 * hstep)  # fourth-order central difference
             e = abs(g[name][k, j, i] - fd) / abs(fd)
             e64 = abs(g_r64[name][k, j, i] - fd) / abs(fd)
             worst = max(worst, e)
@@ -317,7 +322,7 @@ def adjoint_shot(repeats=2, fd_cells=2):
                         "residual_rel_to_traces": float(np.linalg.norm(res64) / np.linalg.norm(obs64)),
                         "fp32_residual_rel_error": res_err,
                         "normwise_vs_fp64_adjoint": normwise,
-                        "note": "FP32 states, FP32 adjoint, FP64 gradient sums against second-order central "
+                        "note": "FP32 states, FP32 adjoint, FP64 gradient sums against fourth-order central "
                                 "differences of the FP64 forward misfit (step 1e-3 of the value), with the "
                                 "residual of the FP32 forward (rel) and of the FP64 forward "
                                 "(rel_fp64_residual); normwise: against the FP64 adjoint of the FP64 residual"}
