@@ -130,9 +130,11 @@ def test_half_precision_full_size_perturbed512():
     """The 512 x 512 x 384, 3000-step perturbed-medium workload at full size, on two receiver
     sets in one run: the 128 x 128 surface receivers (vx, vz, p) and the reference's own layout,
     the vertical line through the source column (kernel.cu:2027-2031; the source cell left
-    out) recording txx.  FP16C (binary16 + 8-bit residual on v, tau, p) and FP16T (residual on
-    tau, p) and FP16M (tau, p binary16 + a packed 4-bit residual; binary16 v, q and moduli) within 1e-2 of the GPU FP64
-    path on both; plain FP16 is reported (it misses 1e-2: 2.0e-2 measured)."""
+    out), which records the reference's txx and here also vx, vz and p.  FP16C (binary16 +
+    8-bit residual on v, tau, p), FP16T (8-bit residual on tau, p) and FP16M (tau, p binary16
+    + a packed 4-bit residual; binary16 v, q and moduli) within 1e-2 of the GPU FP64 path on
+    every trace of both sets (the near-source pressure is the hardest: each step's increment
+    is small against its value); plain FP16 is reported (it misses 1e-2: 2.0e-2 measured)."""
     import time
 
     from biot_amd._lib import FIELD_INDEX
@@ -161,7 +163,7 @@ def test_half_precision_full_size_perturbed512():
     surf = {p: [rel_l2(traces[p][ci[f], :, :nsurf], ref[ci[f], :, :nsurf]) for f in ("vx", "vz", "p")]
             for p in ("fp16c", "fp16t", "fp16m", "fp16")}
     cols = {p: {f: rel_l2(traces[p][ci[f], :, nsurf:], ref[ci[f], :, nsurf:]) for f in fields} for p in surf}
-    col = {p: cols[p]["txx"] for p in surf}
+    col = {p: max(cols[p].values()) for p in surf}
     for p in surf:
         print(f"  {p} surface traces rel L2 vs fp64 (vx, vz, p) = " + " ".join(f"{x:.3e}" for x in surf[p]) +
               "; source column (txx, vx, vz, p) = " +
