@@ -318,7 +318,8 @@ int biot_handle::allocate()
     device_bytes += (int64_t)bytes;
     return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
   };
-  // FP16M keeps the medium records (8 + 16 bytes per cell) in place of the coefficient arrays
+  // FP16M keeps its medium (8 + 8 bytes of records and 2 of mu per cell) in place of the
+  // coefficient arrays
   const size_t coef_bytes = prec == BIOT_FP16M ? mu_offset() + (size_t)2 * n : (size_t)NKC * n * esz;
   const size_t pack_bytes = prec == BIOT_FP16M ? 0 : (size_t)NKC * n * esz;
   bool ok = alloc(&d_fields, fields_bytes()) && alloc(&d_coef, coef_bytes) &&
@@ -684,7 +685,6 @@ namespace {
 enum { MD_M = 0, MD_HH, MD_H2U, MD_C, MD_C1, MD_C2, MD_RHO, MD_RHOF, MD_MU };
 struct MedCtx {
   const std::vector<float> *mod;
-  int nx, ny, nz;
   double sc[9];        // scales: rho rho_f C1 C2 | C M H2u HH | mu
   uint16_t *mv, *mt, *mu;
   double mx[64][9];    // per-thread maxima (first pass)
@@ -730,7 +730,6 @@ int biot_handle::upload_medium()
     }
   auto cx = std::make_unique<MedCtx>();
   cx->mod = mod;
-  cx->nx = g.nx, cx->ny = g.ny, cx->nz = g.nz;
   std::memset(cx->mx, 0, sizeof cx->mx);
   parallel_for(n, med_max_body, cx.get());
   double *sc = cx->sc;

@@ -317,7 +317,7 @@ __device__ __forceinline__ typename P::A dfw(const AT &a, u32 c, u32 s, typename
     return L<P, F>(a, c + s) - fc;
   } else {
     if constexpr (outer16<P, F>) {
-      // FP16M: both outer taps (c + 2s, c - s) as binary16, each an fma on the binary16 value
+      // FP16M, FP16T: both outer taps (c + 2s, c - s) as binary16, each an fma on the binary16 value
       const float o2 = (float)at<_Float16>(a.f[F], c + 2 * s), o1 = (float)at<_Float16>(a.f[F], c - s);
       return fmaf(A(-1.0 / 24.0), o2, fmaf(A(1.0 / 24.0), o1, A(9.0 / 8.0) * (L<P, F>(a, c + s) - fc)));
     } else {
@@ -335,7 +335,7 @@ __device__ __forceinline__ typename P::A dbw(const AT &a, u32 c, u32 s, typename
     return fc - L<P, F>(a, c - s);
   } else {
     if constexpr (outer16<P, F>) {
-      // FP16M: both outer taps (c + s, c - 2s) as binary16, each an fma on the binary16 value
+      // FP16M, FP16T: both outer taps (c + s, c - 2s) as binary16, each an fma on the binary16 value
       const float o1 = (float)at<_Float16>(a.f[F], c + s), o2 = (float)at<_Float16>(a.f[F], c - 2 * s);
       return fmaf(A(-1.0 / 24.0), o1, fmaf(A(1.0 / 24.0), o2, A(9.0 / 8.0) * (fc - L<P, F>(a, c - s))));
     } else {
