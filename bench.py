@@ -553,7 +553,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the 3000-step FP16M accuracy run of the 512 grid")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-graph-steps", type=int, default=50,
