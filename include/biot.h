@@ -61,8 +61,8 @@ enum biot_precision {
                      form the staggered averages, the harmonic means of mu and the
                      coefficients; float C-PML
                      memory and arithmetic.  Needs the moduli: biot_set_medium,
-                     biot_set_medium_device or biot_set_params (not biot_set_coefficients); no
-                     adjoint or batches. */
+                     biot_set_medium_device or biot_set_params (not biot_set_coefficients); the
+                     adjoint after biot_set_params; no batches. */
 };
 
 enum biot_field {
@@ -215,8 +215,12 @@ typedef struct biot_params {
 } biot_params;
 int biot_set_params(biot_handle *h, const biot_params *p);
 /* Resets, runs nt steps recording traces (receivers must hold >= nt steps) and keeps the
- * state every `every` steps on the device for the adjoint.  FP32, FP64, FP16, FP16T and FP16C
- * handles (FP16T and FP16C states carry their residuals, and the adjoint reads them).
+ * state every `every` steps on the device for the adjoint.  Every policy (FP16T and FP16C
+ * states carry their residual arrays, FP16M states its residual records, and the adjoint
+ * reads them).  An FP16M handle needs biot_set_params first (BIOT_ERR_STATE otherwise): it
+ * keeps moduli, not coefficient arrays, so the adjoint reads the parameters' coefficients
+ * as FP16T stores them, binary16 packed per cell, built here (44 bytes per cell, freed with
+ * the checkpoints).
  * every = 1 keeps every step's state (nt + 1 states in one device block, BIOT_ERR_NOMEM if
  * it does not fit): biot_adjoint then reads them instead of recomputing each segment, and
  * gives the same gradients as every = 50 bit for bit. */

@@ -19,7 +19,7 @@ def _np(t):
 
 class BiotTraces(torch.autograd.Function):
     """Inputs: the six inverted parameter tensors (nz, ny, nx), then the solver (a Solver
-    with source and receivers set, FP32 or FP64), the fixed arrays (dict with C1, tphi),
+    with source and receivers set, any storage policy), the fixed arrays (dict with C1, tphi),
     nt and the checkpoint interval.  Output: traces (ncomp, nt_max, nrec), float64 for an
     FP64 solver, else float32."""
 

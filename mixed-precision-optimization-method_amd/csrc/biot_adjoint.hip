@@ -62,6 +62,7 @@ struct AdjT {
   using T = S;  // stored element of fields and coefficients
   static constexpr bool scaled = std::is_same<S, _Float16>::value;
   static constexpr bool res = false;
+  static constexpr bool pk = false;
 };
 // FP16T / FP16C states: binary16 fields and coefficients, an int8 residual on tau and p
 // (FP16C: also on v; value = h + r ulp(h)/256, as the forward kernels store it), float C-PML
@@ -74,6 +75,20 @@ struct AdjT<H16R> {
   using T = _Float16;
   static constexpr bool scaled = true;
   static constexpr bool res = true;
+  static constexpr bool pk = false;
+};
+// FP16M states: binary16 fields; tau and p completed by the 4-bit residual of the cell's
+// 4-byte record (value = h + r ulp(h)/16, as the forward kernels store it); float C-PML
+// carries.  Its coefficients are read as FP16T stores them (binary16, packed per cell).
+struct H16P {};
+template <>
+struct AdjT<H16P> {
+  using A = float;
+  using C = float;
+  using T = _Float16;
+  static constexpr bool scaled = true;
+  static constexpr bool res = false;
+  static constexpr bool pk = true;
 };
 // the residual unit ulp(h)/256 = 2^(e-33), e the biased exponent of h (subnormals e = 1)
 __device__ __forceinline__ float res_unit(_Float16 h)
@@ -111,11 +126,14 @@ __device__ __forceinline__ const void *fw_carry(const AT &a, int q, int ax)
 {
   return a.slots + (uint64_t)q * a.sb + a.coff[ax];
 }
-// the residuals of field f in state q (FP16T tau and p, FP16C also v), or null
-template <class AT>
+// the residuals of field f in state q (FP16T tau and p, FP16C also v), or null; for FP16M
+// the state's record array (256-byte aligned) with the field's nibble in the low 3 bits
+template <typename S, class AT>
 __device__ __forceinline__ const int8_t *fw_lo(const AT &a, int q, int f)
 {
   if (!a.loff || !((a.lomask >> f) & 1u)) return nullptr;
+  if constexpr (AdjT<S>::pk)
+    return reinterpret_cast<const int8_t *>(a.slots + (uint64_t)q * a.sb + a.loff + (uint64_t)pk_slot(f));
   return reinterpret_cast<const int8_t *>(a.slots + (uint64_t)q * a.sb + a.loff +
                                           (uint64_t)f * (uint64_t)a.nx * a.ny * a.nz);
 }
@@ -129,6 +147,13 @@ __device__ __forceinline__ typename AdjT<S>::A diff(const void *f, const int8_t 
     const T h = at<T>(f, idx);
     if constexpr (AdjT<S>::res) {
       if (lo) return (A)h + (A)lo[idx] * res_unit(h);
+    }
+    if constexpr (AdjT<S>::pk) {
+      if (lo) {
+        const uintptr_t t = reinterpret_cast<uintptr_t>(lo);
+        const int sl = (int)(t & 7u);
+        return (A)pk_join((float)h, pk_nib(at<u32>(reinterpret_cast<const void *>(t - sl), idx), sl));
+      }
     }
     return (A)h;
   };
@@ -247,7 +272,7 @@ __device__ __forceinline__ void pml_load(const AT &a, const Der &D, const CellId
 {
   using A = typename AdjT<S>::A;
   const u32 st[3] = {1u, (u32)a.sy, (u32)a.sz};
-  r.E = diff<S, ORDER>(fw_field(a, qf, D.field), fw_lo(a, qf, D.field), ci.c, st[D.dir], D.fwd);
+  r.E = diff<S, ORDER>(fw_field(a, qf, D.field), fw_lo<S>(a, qf, D.field), ci.c, st[D.dir], D.fwd);
   r.ix = carry_idx(a, D.dir, D.slot, ci);
   r.e = r.aa = r.cf = r.lc = A(0);
   if (r.ix >= 0) {
@@ -743,6 +768,8 @@ hipError_t passes(int prec, int order, F f, int first, int last)
     else if (prec == PREC_FP16) e = order == 2 ? f.template run<_Float16, 2>(which) : f.template run<_Float16, 4>(which);
     else if (prec == PREC_FP16T || prec == PREC_FP16C)
       e = order == 2 ? f.template run<H16R, 2>(which) : f.template run<H16R, 4>(which);
+    else if (prec == PREC_FP16M)
+      e = order == 2 ? f.template run<H16P, 2>(which) : f.template run<H16P, 4>(which);
     else return hipErrorInvalidValue;
     if (e != hipSuccess) return e;
   }
@@ -780,7 +807,8 @@ hipError_t launch_adjoint_inject(int prec, const InjectArgs &a, int it, hipStrea
   if (a.nrec <= 0 || a.ncomp <= 0) return hipSuccess;
   dim3 grid((unsigned)((a.nrec + 255) / 256), (unsigned)a.ncomp);
   if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject<double>, grid, dim3(256), 0, s, a, it);
-  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T || prec == PREC_FP16C)
+  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T || prec == PREC_FP16C ||
+           prec == PREC_FP16M)
     hipLaunchKernelGGL(k_adj_inject<float>, grid, dim3(256), 0, s, a, it);
   else return hipErrorInvalidValue;
   return hipGetLastError();
@@ -792,7 +820,8 @@ hipError_t launch_adjoint_inject_batch(int prec, const InjectArgs *tab, int nsho
   if (nshots <= 0 || max_nrec <= 0 || max_ncomp <= 0) return hipSuccess;
   dim3 grid((unsigned)((max_nrec + 255) / 256), (unsigned)max_ncomp, (unsigned)nshots);
   if (prec == PREC_FP64) hipLaunchKernelGGL(k_adj_inject_batch<double>, grid, dim3(256), 0, s, tab, it);
-  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T || prec == PREC_FP16C)
+  else if (prec == PREC_FP32 || prec == PREC_FP16 || prec == PREC_FP16T || prec == PREC_FP16C ||
+           prec == PREC_FP16M)
     hipLaunchKernelGGL(k_adj_inject_batch<float>, grid, dim3(256), 0, s, tab, it);
   else return hipErrorInvalidValue;
   return hipGetLastError();

@@ -93,8 +93,6 @@ extern "C" int biot_set_params(biot_handle *h, const biot_params *p)
 extern "C" int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t every)
 {
   if (!h) return BIOT_ERR_ARG;
-  if (h->prec == BIOT_FP16M)
-    return afail(h, BIOT_ERR_ARG, "the adjoint path runs on FP32, FP64, FP16, FP16T or FP16C handles");
   if (nt <= 0 || every <= 0) return afail(h, BIOT_ERR_ARG, "nt and every must be positive");
   if (!h->have_coef) return afail(h, BIOT_ERR_STATE, "no medium set");
   if (h->nt_max < nt || h->ncomp == 0) return afail(h, BIOT_ERR_STATE, "receivers must record >= nt steps");
@@ -102,6 +100,7 @@ extern "C" int biot_forward_checkpointed(biot_handle *h, int32_t nt, int32_t eve
     if ((h->src_mask >> f) & 1u && !(f >= 6))
       return afail(h, BIOT_ERR_ARG, "the adjoint path supports sources on stresses and pressure only");
   h->free_adjoint();
+  if (int rc = h->build_adjoint_coefs()) return rc;  // FP16M: the coefficient copy the adjoint reads
   const size_t sb = h->state_bytes();
   if (every == 1) {
     // every step's state, in one block the adjoint reads directly (no recomputation)

@@ -62,6 +62,22 @@ constexpr int pk_slot(int f)
 }
 // the kernels take field PK_FIRST's residual pointer as the record base
 static_assert(pk_slot(PK_FIRST) == 0, "field PK_FIRST must own nibble 0 of the record");
+// ulp(h)/16 from h as a float (exponent below binary16's normal range: the subnormal ulp)
+__device__ __forceinline__ float pk_unit(float hf)
+{
+  const uint32_t e = (__builtin_bit_cast(uint32_t, hf) >> 23) & 0xFFu;
+  return __builtin_bit_cast(float, (max(e, 113u) - 14u) << 23);
+}
+// residual s (0..6) of a record, sign-extended
+__device__ __forceinline__ int pk_nib(uint32_t r, int s)
+{
+  return (int)(r << (28 - 4 * s)) >> 28;
+}
+// h as a float plus residual n: the value the forward kernels and the adjoint read
+__device__ __forceinline__ float pk_join(float hf, int n)
+{
+  return fmaf((float)n, pk_unit(hf), hf);
+}
 // kernel coefficient order
 enum {
   K_U1X = 0, K_U1Y, K_U1Z, K_U2X, K_U2Y, K_U2Z, K_W1X, K_W1Y, K_W1Z, K_W2X, K_W2Y, K_W2Z,

@@ -151,8 +151,8 @@ void biot_handle::adj_args(AdjArgs &A) const
   A.slots = (const char *)d_slots;
   A.sb = state_bytes();
   A.fbytes = (uint64_t)n * esz;
-  // FP16T / FP16C residuals (FP16M never reaches the adjoint)
-  const bool res = prec == BIOT_FP16T || prec == BIOT_FP16C;
+  // FP16T / FP16C residual arrays, FP16M's residual records (tau and p)
+  const bool res = prec == BIOT_FP16T || prec == BIOT_FP16C || prec == BIOT_FP16M;
   A.loff = res ? state_lo_off() : 0;
   A.lomask = res ? comp_fields(prec) : 0u;
   for (int ax = 0; ax < 3; ++ax) A.coff[ax] = state_off(1 + ax);
@@ -167,7 +167,8 @@ void biot_handle::adj_args(AdjArgs &A) const
   for (int k = 0; k < NKC; ++k) {
     A.grad[k] = d_grad + (size_t)k * n;
     A.gacc[k] = prec == BIOT_FP64 ? (void *)A.grad[k] : (void *)(d_gpart + (size_t)k * n);
-    A.k[k] = (const char *)d_coef + (size_t)k * n * esz;
+    // (FP16M keeps no per-array coefficients; the adjoint reads the packed copy only)
+    A.k[k] = prec == BIOT_FP16M ? nullptr : (const char *)d_coef + (size_t)k * n * esz;
     A.ks[k] = ks[k];
   }
   A.gpv = prec == BIOT_FP64 ? nullptr : (void *)d_gpart;
@@ -211,6 +212,10 @@ void biot_handle::free_adjoint()
     if (p) (void)hipFree(p);
   ckpt.clear();
   ckpt_every = ckpt_nt = 0;
+  if (prec == BIOT_FP16M && d_kpack) {  // the adjoint's coefficient copy of the last parameters
+    (void)hipFree(d_kpack);
+    d_kpack = nullptr;
+  }
   for (void **p : {&d_slots, &d_lam, (void **)&d_grad, (void **)&d_gpart, (void **)&d_adjsrc})
     if (*p) {
       (void)hipFree(*p);
@@ -602,6 +607,51 @@ int biot_handle::upload_from_ref(const float *d_ref)
   if ((rc = pack_coefficients())) return rc;
   have_coef = true;
   return BIOT_OK;
+}
+
+// FP16M keeps moduli, not coefficient arrays: the adjoint reads the coefficients of the last
+// parameter set as FP16T stores them (K kfac / ks rounded to binary16, packed per cell).  The
+// forward forms them from the binary16 moduli in float, so the two differ by the rounding of
+// either.  Built when a checkpointed forward run starts, freed with the adjoint buffers.
+int biot_handle::build_adjoint_coefs()
+{
+  if (prec != BIOT_FP16M || d_kpack) return BIOT_OK;
+  if (K_last[0].empty()) {
+    err = "FP16M gradients need biot_set_params (its coefficients come from the parameters)";
+    return BIOT_ERR_STATE;
+  }
+  const size_t bytes = (size_t)NKC * n * sizeof(_Float16);
+  void *soa = nullptr;
+  if (hipMalloc(&soa, bytes) != hipSuccess) {
+    err = "the adjoint's coefficient arrays do not fit in device memory";
+    return BIOT_ERR_NOMEM;
+  }
+  if (hipMalloc(&d_kpack, bytes) != hipSuccess) {
+    d_kpack = nullptr;
+    (void)hipFree(soa);
+    err = "the adjoint's coefficient arrays do not fit in device memory";
+    return BIOT_ERR_NOMEM;
+  }
+  std::vector<_Float16> buf((size_t)n);
+  int rc = BIOT_OK;
+  for (int k = 0; k < NKC && !rc; ++k) {
+    const double f = kfac[k] / (double)ks[k];
+    for (int64_t c = 0; c < n; ++c) buf[(size_t)c] = (_Float16)(float)(K_last[k][(size_t)c] * f);
+    if (hipMemcpy((char *)soa + (size_t)k * n * sizeof(_Float16), buf.data(), (size_t)n * sizeof(_Float16),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      rc = BIOT_ERR_HIP;
+  }
+  if (!rc && (launch_pack_coef(PREC_FP16T, soa, d_kpack, (char *)d_kpack + (size_t)14 * n * sizeof(_Float16),
+                               n, stream) != hipSuccess ||
+              hipStreamSynchronize(stream) != hipSuccess))
+    rc = BIOT_ERR_HIP;
+  (void)hipFree(soa);
+  if (rc) {
+    (void)hipFree(d_kpack);
+    d_kpack = nullptr;
+    err = "building the adjoint's coefficient copy failed";
+  }
+  return rc;
 }
 
 int biot_handle::pack_coefficients()

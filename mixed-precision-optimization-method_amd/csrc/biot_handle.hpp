@@ -89,6 +89,7 @@ struct biot_handle {
   int upload_profiles();
   int upload_coefficients();
   int pack_coefficients();           // d_coef -> d_kpack
+  int build_adjoint_coefs();         // FP16M: d_kpack for the adjoint, from K_last
   int upload_kernel_coefs(const std::vector<double> (&K)[biot::NKC]);
   int upload_from_ref(const float *d_ref);      // 23 reference arrays on the device
   int upload_medium();                          // FP16M: mod -> medium records, mc

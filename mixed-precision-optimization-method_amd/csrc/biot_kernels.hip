@@ -127,12 +127,6 @@ __device__ __forceinline__ float res_inv(int e) { return __builtin_bit_cast(floa
 // is 256-byte aligned, so a field's residual pointer carries the field's nibble in its low 3
 // bits (the runtime loaders below).
 typedef uint32_t dw2_t __attribute__((ext_vector_type(2)));
-// ulp(h)/16 from h as a float (exponent below binary16's normal range: the subnormal ulp)
-__device__ __forceinline__ float pk_unit(float hf)
-{
-  const u32 e = (__builtin_bit_cast(u32, hf) >> 23) & 0xFFu;
-  return __builtin_bit_cast(float, (max(e, 113u) - 14u) << 23);
-}
 // h and r of x: r as its nibble (two's complement)
 __device__ __forceinline__ void pk_split(float x, uint16_t &hb, u32 &rb)
 {
@@ -142,16 +136,6 @@ __device__ __forceinline__ void pk_split(float x, uint16_t &hb, u32 &rb)
   const float inv = __builtin_bit_cast(float, (u32)(156 - h_exp(h)) << 23);
   const float r = rintf((x - (float)h) * inv);
   rb = (u32)(int)fminf(fmaxf(r, -7.0f), 7.0f) & 0xFu;
-}
-// residual s (0..6) of a record, sign-extended
-__device__ __forceinline__ int pk_nib(u32 r, int s)
-{
-  return (int)(r << (28 - 4 * s)) >> 28;
-}
-// h as a float plus residual n
-__device__ __forceinline__ float pk_join(float hf, int n)
-{
-  return fmaf((float)n, pk_unit(hf), hf);
 }
 // the binary16 part alone, as a float
 __device__ __forceinline__ float pk_hi(const void *hi, u32 idx)

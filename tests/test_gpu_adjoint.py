@@ -12,6 +12,7 @@
 * The torch.autograd binding returns the same gradients as the direct call.
 * Every step's state stored (every = 1, no segment recomputation) gives the gradients of
   50-step checkpoints bit for bit.
+* FP16M's gradients (packed residual records in its states) against the FP32 ones.
 """
 import math
 
@@ -21,6 +22,7 @@ import torch
 
 from biot_amd import BiotError, autograd, configs, media
 from biot_amd._lib import PARAM_NAMES
+from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -173,7 +175,7 @@ def test_fp64_adjoint_matches_host_adjoint():
     assert max(eg.values()) <= 1e-8
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp16t", "fp16c", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp16t", "fp16c", "fp16m", "fp64"])
 def test_stored_states_equal_recomputed_segments(precision):
     cfg = configs.adjoint128(n=48, nt=260, every=50, sphere=False)
     nt = cfg["nt"]
@@ -198,6 +200,59 @@ def test_stored_states_equal_recomputed_segments(precision):
         assert np.array_equal(a, b)
     assert any(np.abs(out[1][1][k]).max() > 0 for k in PARAM_NAMES)
     s.close()
+
+
+def test_fp16m_gradient_against_fp32():
+    """FP16M gradients against the FP32 ones, FP16T (the 16-bit policy the batched gradients
+    use) beside them, on the 64^3 start model, 400 steps, 50-step checkpoints.  FP16M's
+    checkpoints carry its residual records, the adjoint reads tau and p with their 4-bit
+    residuals and the coefficients as FP16T stores them (binary16 from the parameters; the
+    FP16M forward forms them in float from binary16 moduli).  Two misfits:
+
+    * energy: observed traces zero, so the residual is the data and the 16-bit traces' own
+      error stays small beside it.  Measured on MI355X: J 1.7e-5 from FP32; lambda, alpha, M
+      1.4e-5, mu 3.6e-4, rho 8.7e-4, rho_f 2.8e-3 (FP16T 3.6e-5, 6.8e-4, 7.3e-4, 7.8e-3);
+      held to 1e-4 (J; lambda, alpha, M), 3e-3 (mu, rho) and 1e-2 (rho_f);
+    * model: observed traces from the FP64 run of a model with lambda +10 % and mu -5 %
+      everywhere.  Its residual is a small part of the data late in the run, and both 16-bit
+      policies sit well above their batched-gradient levels there (FP16T measured 3.4e-3 on
+      lambda, 8e-2 on mu, 0.16 on rho_f; FP16M 8.4e-4, 4.3e-2, 5.8e-2, and 4.1e-2 on rho
+      against FP16T's 3.5e-2).  FP16M is held to FP16T's level: within 2x per parameter and J
+      within 2x FP16T's J error (plus 1 %)."""
+    cfg = configs.adjoint128(n=64, nt=400, every=50, sphere=False)
+    nt, every = cfg["nt"], cfg["every"]
+    p0 = _start_params(cfg)
+    p_true = {k: v.copy() for k, v in p0.items()}
+    p_true["lambda"] *= 1.10
+    p_true["mu"] *= 0.95
+    s64 = configs.make_solver(cfg, precision="fp64")
+    obs_model = _forward_f64(s64, p_true, nt).astype(np.float32)
+    s64.close()
+    res = {}
+    for case, obs in (("energy", np.zeros_like(obs_model)), ("model", obs_model)):
+        print(f"fp16m / fp16t gradients vs fp32, 64^3 400 steps, {case} misfit", flush=True)
+        out = {}
+        for prec in ("fp32", "fp16t", "fp16m"):
+            s = configs.make_solver(cfg, precision=prec)
+            out[prec] = autograd.misfit_gradient(p0, s, obs, nt, every)
+            s.close()
+        J32, g32 = out["fp32"]
+        for prec in ("fp16t", "fp16m"):
+            J, g = out[prec]
+            assert all(np.isfinite(g[k]).all() for k in PARAM_NAMES), (case, prec)
+            e = {k: rel_l2(g[k], g32[k]) for k in PARAM_NAMES}
+            res[case, prec] = (abs(J / J32 - 1), e)
+            print(f"  {prec}: J/J32 - 1 = {J / J32 - 1:.2e}; gradient rel L2 " +
+                  " ".join(f"{k} {x:.2e}" for k, x in e.items()), flush=True)
+    dJ, e = res["energy", "fp16m"]
+    assert dJ <= 1e-4
+    assert max(e[k] for k in ("lambda", "alpha", "M")) <= 1e-4
+    assert max(e["mu"], e["rho"]) <= 3e-3
+    assert e["rhof"] <= 1e-2
+    (dJm, em), (dJt, et) = res["model", "fp16m"], res["model", "fp16t"]
+    assert dJm <= 2 * dJt + 1e-2
+    for k in PARAM_NAMES:
+        assert em[k] <= 2 * et[k] + 1e-3, k
 
 
 def test_autograd_binding_matches_direct_adjoint():
@@ -230,7 +285,7 @@ def test_adjoint_argument_errors():
     with pytest.raises(BiotError):  # more steps than the receivers hold
         s.forward_checkpointed(cfg["nt"] + 1, 10)
     s16 = configs.make_solver(cfg, precision="fp16m")
-    with pytest.raises(BiotError):  # FP16M holds no coefficient arrays for the adjoint
+    with pytest.raises(BiotError):  # FP16M's adjoint coefficients come from biot_set_params
         s16.forward_checkpointed(cfg["nt"], 10)
     s16.close()
     s.forward_checkpointed(cfg["nt"], 10)
