@@ -62,7 +62,7 @@ enum biot_precision {
                      coefficients; float C-PML
                      memory and arithmetic.  Needs the moduli: biot_set_medium,
                      biot_set_medium_device or biot_set_params (not biot_set_coefficients); the
-                     adjoint after biot_set_params; no batches. */
+                     adjoint after biot_set_params. */
 };
 
 enum biot_field {
@@ -266,7 +266,7 @@ int biot_params_chain(const biot_grid *g, const biot_params *p, const double *co
  * captured into a HIP graph and replayed.  The gradient sums every shot's parameter
  * gradient into one model-grid volume on the device.  Precisions FP32, FP16 (binary16
  * wavefields, float arithmetic, FP64 gradient sums), FP16T and FP16C (FP16 with the 8-bit
- * residual on tau and p, and on v too), FP64. */
+ * residual on tau and p, and on v too), FP16M, FP64. */
 typedef struct biot_batch biot_batch;
 typedef struct biot_batch_config {
   int32_t nx, ny, nz;        /* model grid the windows are cut from (gradient volume) */

@@ -173,8 +173,9 @@ extern "C" int biot_batch_create(const biot_batch_config *c, biot_batch **out)
     return refuse(nullptr, BIOT_ERR_ARG, "the model grid must be at least 8 cells on every axis");
   if (c->order != 2 && c->order != 4) return refuse(nullptr, BIOT_ERR_ARG, "order must be 2 or 4");
   if (c->precision != BIOT_FP32 && c->precision != BIOT_FP16 && c->precision != BIOT_FP16T &&
-      c->precision != BIOT_FP16C && c->precision != BIOT_FP64)
-    return refuse(nullptr, BIOT_ERR_ARG, "batches run BIOT_FP32, BIOT_FP16, BIOT_FP16T, BIOT_FP16C or BIOT_FP64");
+      c->precision != BIOT_FP16C && c->precision != BIOT_FP16M && c->precision != BIOT_FP64)
+    return refuse(nullptr, BIOT_ERR_ARG,
+                  "batches run BIOT_FP32, BIOT_FP16, BIOT_FP16T, BIOT_FP16C, BIOT_FP16M or BIOT_FP64");
   if (!(c->dx > 0) || !(c->dt > 0) || c->npml < 0)
     return refuse(nullptr, BIOT_ERR_ARG, "dx and dt must be positive and npml >= 0");
   int ndev = 0;
@@ -539,6 +540,7 @@ size_t shot_gradient_bytes(const biot_handle *h, int nseg, int every)
   const int64_t m = (int64_t)h->ncomp * h->nt_max * h->nrec;
   return h->state_bytes() * (size_t)(nseg + every + 1) + h->lam_bytes() +
          (size_t)NKC * h->n * (sizeof(double) + (h->prec == BIOT_FP64 ? 0 : sizeof(float))) +
+         (h->prec == BIOT_FP16M ? (size_t)NKC * h->n * sizeof(_Float16) : 0) +  // adjoint coefficients
          (size_t)8 * h->n * sizeof(double) +
          (size_t)2 * std::max<int64_t>(m, 1) * sizeof(float);
 }
@@ -556,6 +558,7 @@ int gradient_group(biot_batch *b, int s0, int s1, int nt, int every, const float
     auto &s = b->shots[i];
     biot_handle *h = s.h;
     if ((rc = h->alloc_adjoint(every))) return hfail(b, h, rc, "adjoint buffers");
+    if ((rc = h->build_adjoint_coefs())) return hfail(b, h, rc, "adjoint coefficients");
     for (int k = 0; k < nseg; ++k) {
       void *p = nullptr;
       if (hipMalloc(&p, h->state_bytes()) != hipSuccess)

@@ -1167,6 +1167,9 @@ hipError_t launch_step_batch(int prec, int order, bool vel, const BatchArgs &b, 
     case PREC_FP16T:
       if (order == 2) { BIOT_BATCH_CASE(P16T, 2) } else { BIOT_BATCH_CASE(P16T, 4) }
       break;
+    case PREC_FP16M:
+      if (order == 2) { BIOT_BATCH_CASE(P16M, 2) } else { BIOT_BATCH_CASE(P16M, 4) }
+      break;
     case PREC_FP64:
       if (order == 2) { BIOT_BATCH_CASE(P64, 2) } else { BIOT_BATCH_CASE(P64, 4) }
       break;
@@ -1184,6 +1187,7 @@ hipError_t launch_source_batch(int prec, const SourceArgs *tab, int nshots, cons
     case PREC_FP16: hipLaunchKernelGGL(k_source_batch<P16>, dim3(nshots), dim3(64), 0, s, tab, it); break;
     case PREC_FP16C: hipLaunchKernelGGL(k_source_batch<P16C>, dim3(nshots), dim3(64), 0, s, tab, it); break;
     case PREC_FP16T: hipLaunchKernelGGL(k_source_batch<P16T>, dim3(nshots), dim3(64), 0, s, tab, it); break;
+    case PREC_FP16M: hipLaunchKernelGGL(k_source_batch<P16M>, dim3(nshots), dim3(64), 0, s, tab, it); break;
     case PREC_FP64: hipLaunchKernelGGL(k_source_batch<P64>, dim3(nshots), dim3(64), 0, s, tab, it); break;
     default: return hipErrorInvalidValue;
   }
@@ -1200,6 +1204,7 @@ hipError_t launch_record_batch(int prec, const RecordArgs *tab, int nshots, int6
     case PREC_FP16: hipLaunchKernelGGL(k_record_batch<P16>, grid, block, 0, s, tab, it); break;
     case PREC_FP16C: hipLaunchKernelGGL(k_record_batch<P16C>, grid, block, 0, s, tab, it); break;
     case PREC_FP16T: hipLaunchKernelGGL(k_record_batch<P16T>, grid, block, 0, s, tab, it); break;
+    case PREC_FP16M: hipLaunchKernelGGL(k_record_batch<P16M>, grid, block, 0, s, tab, it); break;
     case PREC_FP64: hipLaunchKernelGGL(k_record_batch<P64>, grid, block, 0, s, tab, it); break;
     default: return hipErrorInvalidValue;
   }

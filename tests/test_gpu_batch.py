@@ -39,7 +39,7 @@ def _single(cfg, shot, precision):
     return s
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp16t", "fp16c"])
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp16t", "fp16c", "fp16m"])
 def test_batch_forward_matches_single_grids_eager_and_graph(precision):
     print(f"batch forward {precision}", flush=True)
     cfg = _small()

@@ -168,8 +168,8 @@ def test_batch_create_reports_argument_errors_through_null_handle():
     base = dict(nx=32, ny=32, nz=32, order=4, precision=1, npml=4, dx=5, dt=4e-4, pml_v0=4000,
                 pml_R=1e-6, Cv=1e8, Cp=1, device=0)
     cases = [(dict(nx=7), "at least 8 cells"), (dict(order=3), "order must be 2 or 4"),
-             (dict(precision=5), "BIOT_FP32, BIOT_FP16, BIOT_FP16T, BIOT_FP16C or BIOT_FP64"),
-             (dict(precision=6), "BIOT_FP32, BIOT_FP16, BIOT_FP16T, BIOT_FP16C or BIOT_FP64"),
+             (dict(precision=-1), "BIOT_FP32, BIOT_FP16, BIOT_FP16T, BIOT_FP16C, BIOT_FP16M or BIOT_FP64"),
+             (dict(precision=6), "BIOT_FP32, BIOT_FP16, BIOT_FP16T, BIOT_FP16C, BIOT_FP16M or BIOT_FP64"),
              (dict(dt=0.0), "dx and dt must be positive"), (dict(npml=-1), "npml >= 0")]
     for change, why in cases:
         c = BatchConfig(**{**base, **change})
