@@ -923,30 +923,39 @@ __global__ void __launch_bounds__(256) k_step_batch(const BatchArgs b)
     if (VEL) velocity_tile<P, ORDER, false>(a, tx, ty, tz);
     else stress_tile<P, ORDER, false>(a, tx, ty, tz);
   };
-  if (!b.queue) {  // fixed mapping: block l of the group takes tiles l, l + per_x, ...
-    const int per_x = gridDim.x >> 3;
-    for (int t = begin + (blockIdx.x >> 3); t < end; t += per_x) tile(t);
-    return;
-  }
-  // Persistent grid with a work queue: a block takes the group's next tile from the group's
-  // counter (one atomic a tile, the next one claimed while the current one runs), so blocks
-  // that drew cheap tiles take more.  Every block ends on one claim past the range; the last
-  // block of the group to make it sets the group's two counters back to zero for the next
-  // launch (no other block of the group touches them after its own last claim).
+  // Both mappings run one loop, so the tile body is inlined once (two inlined copies took the
+  // residual policies' kernels to 248-256 VGPRs).
+  // Fixed mapping (no queue): block l of the group takes tiles l, l + per_x, ...
+  // Work queue: a block takes the group's next tile from the group's counter (one atomic a
+  // tile, the next one claimed while the current one runs), so blocks that drew cheap tiles
+  // take more.  Every block ends on one claim past the range; the last block of the group to
+  // make it sets the group's two counters back to zero for the next launch (no other block
+  // of the group touches them after its own last claim).  The branches on the queue are
+  // uniform over the launch, so the barriers inside them are reached by every thread.
   __shared__ int s_next;
+  const bool queue = b.queue != nullptr;
   const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
-  int *ctr = b.queue + x, *done = b.queue + 8 + x;
-  if (lead) s_next = atomicAdd(ctr, 1);
-  __syncthreads();
-  int t = begin + s_next;
-  while (t < end) {
-    __syncthreads();  // every thread has read s_next
+  int *ctr = queue ? b.queue + x : nullptr, *done = queue ? b.queue + 8 + x : nullptr;
+  int t = begin + (blockIdx.x >> 3);
+  if (queue) {
     if (lead) s_next = atomicAdd(ctr, 1);
-    tile(t);
-    __syncthreads();  // the next claim is in s_next
+    __syncthreads();
     t = begin + s_next;
   }
-  if (lead && atomicAdd(done, 1) == (int)(gridDim.x >> 3) - 1) {
+  while (t < end) {
+    if (queue) {
+      __syncthreads();  // every thread has read s_next
+      if (lead) s_next = atomicAdd(ctr, 1);
+    }
+    tile(t);
+    if (queue) {
+      __syncthreads();  // the next claim is in s_next
+      t = begin + s_next;
+    } else {
+      t += gridDim.x >> 3;
+    }
+  }
+  if (queue && lead && atomicAdd(done, 1) == (int)(gridDim.x >> 3) - 1) {
     atomicExch(ctr, 0);
     atomicExch(done, 0);
   }
