@@ -609,6 +609,24 @@ int biot_handle::upload_from_ref(const float *d_ref)
   return BIOT_OK;
 }
 
+namespace {
+struct AdjCoefCtx {
+  const biot_handle *h;
+  _Float16 *out;  // NKC arrays of n
+};
+void adj_coef_body(int64_t lo, int64_t hi, void *p)
+{
+  auto *cx = static_cast<AdjCoefCtx *>(p);
+  const biot_handle *h = cx->h;
+  for (int k = 0; k < NKC; ++k) {
+    const double f = h->kfac[k] / (double)h->ks[k];
+    const double *K = h->K_last[k].data();
+    _Float16 *o = cx->out + (size_t)k * h->n;
+    for (int64_t c = lo; c < hi; ++c) o[c] = (_Float16)(float)(K[c] * f);
+  }
+}
+}  // namespace
+
 // FP16M keeps moduli, not coefficient arrays: the adjoint reads the coefficients of the last
 // parameter set as FP16T stores them (K kfac / ks rounded to binary16, packed per cell).  The
 // forward forms them from the binary16 moduli in float, so the two differ by the rounding of
@@ -632,15 +650,11 @@ int biot_handle::build_adjoint_coefs()
     err = "the adjoint's coefficient arrays do not fit in device memory";
     return BIOT_ERR_NOMEM;
   }
-  std::vector<_Float16> buf((size_t)n);
-  int rc = BIOT_OK;
-  for (int k = 0; k < NKC && !rc; ++k) {
-    const double f = kfac[k] / (double)ks[k];
-    for (int64_t c = 0; c < n; ++c) buf[(size_t)c] = (_Float16)(float)(K_last[k][(size_t)c] * f);
-    if (hipMemcpy((char *)soa + (size_t)k * n * sizeof(_Float16), buf.data(), (size_t)n * sizeof(_Float16),
-                  hipMemcpyHostToDevice) != hipSuccess)
-      rc = BIOT_ERR_HIP;
-  }
+  // the 22 arrays converted on the host threads, then one upload
+  std::vector<_Float16> buf((size_t)NKC * n);
+  AdjCoefCtx cx{this, buf.data()};
+  parallel_for(n, adj_coef_body, &cx);
+  int rc = hipMemcpy(soa, buf.data(), bytes, hipMemcpyHostToDevice) == hipSuccess ? BIOT_OK : BIOT_ERR_HIP;
   if (!rc && (launch_pack_coef(PREC_FP16T, soa, d_kpack, (char *)d_kpack + (size_t)14 * n * sizeof(_Float16),
                                n, stream) != hipSuccess ||
               hipStreamSynchronize(stream) != hipSuccess))
