@@ -339,7 +339,8 @@ def batch_shots(nt=1000, every=50, graph_steps=50):
     """16 shots in cubic windows of 96..192 cells (seeded) cut from the 512 x 512 x 384
     perturbed medium, FP16 wavefields: the graph-captured batched forward (throughput over
     all windows) and the batched misfit gradient (checkpoints every 50 steps, FP64 sums,
-    per-shot gradients added into the model grid on the device)."""
+    per-shot gradients added into the model grid on the device); then the batched forward
+    on FP16M wavefields."""
     import numpy as np
 
     from biot_amd import configs
@@ -371,6 +372,15 @@ def batch_shots(nt=1000, every=50, graph_steps=50):
     groups = b.gradient_groups()
     finite = bool(all(np.isfinite(v).all() for v in g.values()))
     b.close()
+    # the same forward on FP16M wavefields (binary16 v, q; tau, p with a packed 4-bit
+    # residual), the 16-bit policy within 1e-2 of FP64 on the 512 x 512 x 384 workload
+    bm = configs.make_batch(cfg, precision="fp16m")
+    launch_m = bm.autotune(3)
+    bm.forward(nt, graph_steps=graph_steps)
+    bm.forward(nt, graph_steps=graph_steps)
+    m_ms = bm.last_ms()
+    finite = finite and bool(all(np.isfinite(bm.traces(i)).all() for i in range(len(cfg["shots"]))))
+    bm.close()
     return dict(workload=f"batch16: 16 shots, cubic windows {min(sh['size'][0] for sh in cfg['shots'])}.."
                          f"{max(sh['size'][0] for sh in cfg['shots'])} cells cut from the 512x512x384 "
                          f"perturbed medium, order 4, PML 20, {nt} steps, FP16 wavefields",
@@ -381,7 +391,9 @@ def batch_shots(nt=1000, every=50, graph_steps=50):
                            else f"one block per tile ({launch[3]} blocks)"),
                 persistent_grid_Mcell_updates_per_s=persistent, policy="fp16 (binary16 wavefields)",
                 gradient_ms=round(g_ms, 1), gradient_ms_per_shot=round(g_ms / len(cfg["shots"]), 1),
-                checkpoint_every=every, gradient_groups=groups, misfit=J, finite=finite)
+                checkpoint_every=every, gradient_groups=groups, misfit=J,
+                fp16m_forward_Mcell_updates_per_s=round(cells * nt / m_ms / 1e3, 1),
+                fp16m_launch_bx_by_zchunk_blocks=list(launch_m), finite=finite)
 
 
 def stream_copy_rate(mib=1024):
@@ -633,7 +645,8 @@ def main():
         log("[bench] batched shots ...")
         out["batch"] = batch_shots(graph_steps=args.batch_graph_steps)
         log(f"[bench] batch: forward {out['batch']['forward_Mcell_updates_per_s']:.0f} Mcell/s, "
-            f"gradient {out['batch']['gradient_ms']:.0f} ms")
+            f"gradient {out['batch']['gradient_ms']:.0f} ms, FP16M forward "
+            f"{out['batch']['fp16m_forward_Mcell_updates_per_s']:.0f} Mcell/s")
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
