@@ -16,8 +16,9 @@ namespace biot {
 struct AdjArgs {
   const char *slots;
   uint64_t sb, fbytes, coff[3];
-  uint64_t loff;             // FP16T / FP16C: offset of the residual arrays (int8, field f at f*n) in a state, else 0
-  uint32_t lomask;           // fields whose residual the adjoint reads (FP16T tau, p; FP16C v, tau, p)
+  uint64_t loff;             // offset in a state of the FP16T / FP16C residual arrays (int8, field f at
+                             // f*n) or of FP16M's residual records (4 bytes per cell), else 0
+  uint32_t lomask;           // fields whose residual the adjoint reads (FP16T, FP16M tau, p; FP16C v, tau, p)
   void *lam[NFIELD];         // adjoint wavefields (arithmetic type), storage units
   void *lcarry[3];           // adjoint carries (arithmetic type), forward layout
   void *G[12];               // derivative adjoints of the current half step (arithmetic type)
